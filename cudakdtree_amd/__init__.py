@@ -1,0 +1,293 @@
+"""cudakdtree_amd -- MI355X (gfx950) k-d tree build and query.
+
+Python binding over the C ABI of ``libcukd_amd.so`` (include/cukd_amd.h).
+Device memory and streams come from PyTorch; every call is stream-ordered
+on ``torch.cuda.current_stream()`` unless a stream is passed.
+
+The names mirror the reference's header API (cudaKDTree v1.0.1):
+
+  buildTree / buildTree_thrust / buildTree_bitonic / buildTree_inPlace
+      -> cukd::buildTree* (builder.h:108-151)
+  computeBounds                        -> builder_common.h:59-64
+  stackBased.fcp / stackFree.fcp / cct.fcp      -> fcp.h:61-163
+  stackBased.knn / stackFree.knn / cct.knn      -> knn.h:83-199
+  SpatialKDTree, buildSpatialTree, free         -> spatial-kdtree.h:69-126
+
+Points are a device tensor whose rows are ``layout.stride`` bytes (e.g. a
+float32 [n, 3] tensor for float3, or a uint8/int32 view of a payload
+struct array).  The library must be built (``python -m
+cudakdtree_amd.build``); there is no fallback path.
+"""
+import ctypes
+import math
+import os
+import types
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libcukd_amd.so")
+_lib = None
+
+OK = 0
+ERR_INVALID, ERR_HIP, ERR_NO_BOUNDS, ERR_ALLOC, ERR_UNSUPPORTED = -1, -2, -3, -4, -5
+SCALAR_F32, SCALAR_I32 = 0, 1
+BUILDER_DEFAULT, BUILDER_THRUST, BUILDER_BITONIC, BUILDER_INPLACE = 0, 1, 2, 3
+STACK_BASED, STACK_FREE, CCT = 0, 1, 2
+LIST_FIXED, LIST_HEAP = 0, 1
+QUERY_SORT = 1
+
+# every entry point declared in include/cukd_amd.h
+EXPORTED_SYMBOLS = (
+    "cukd_build_tree", "cukd_build_tree_temp_bytes", "cukd_compute_bounds",
+    "cukd_fcp", "cukd_knn", "cukd_spatial_build", "cukd_spatial_free",
+    "cukd_spatial_fcp", "cukd_spatial_knn", "cukd_set_traversal_stats",
+    "cukd_last_error", "cukd_version",
+)
+
+
+class CukdError(RuntimeError):
+    """Raised for a non-zero status of a C-ABI call (std::runtime_error in
+    the reference, common.h:276-287)."""
+
+    def __init__(self, code, msg):
+        super().__init__("%s (code %d)" % (msg, code))
+        self.code = code
+
+
+class Layout(ctypes.Structure):
+    """Runtime data_traits: see cukd_layout in include/cukd_amd.h."""
+    _fields_ = [("num_dims", ctypes.c_int32), ("scalar_type", ctypes.c_int32),
+                ("stride", ctypes.c_int32), ("point_offset", ctypes.c_int32),
+                ("dim_offset", ctypes.c_int32), ("dim_bytes", ctypes.c_int32)]
+
+    def __init__(self, num_dims, scalar_type=SCALAR_F32, stride=None,
+                 point_offset=0, dim_offset=-1, dim_bytes=4):
+        super().__init__(num_dims, scalar_type,
+                         stride if stride is not None else 4 * num_dims,
+                         point_offset, dim_offset, dim_bytes)
+
+    @property
+    def explicit_dim(self):
+        return self.dim_offset >= 0
+
+    @property
+    def scalar_dtype(self):
+        return np.float32 if self.scalar_type == SCALAR_F32 else np.int32
+
+    def __repr__(self):
+        return ("Layout(num_dims=%d, scalar_type=%d, stride=%d, point_offset=%d, "
+                "dim_offset=%d, dim_bytes=%d)" % (self.num_dims, self.scalar_type,
+                                                  self.stride, self.point_offset,
+                                                  self.dim_offset, self.dim_bytes))
+
+
+def float_points(d):
+    """float2/float3/float4 / vec_float<d> (cukd-math.h:43-181)."""
+    return Layout(d, SCALAR_F32)
+
+
+def int_points(d):
+    """int2/int3/int4 (cukd-math.h:50-52)."""
+    return Layout(d, SCALAR_I32)
+
+
+def point_plus_payload(d, scalar_type=SCALAR_F32, explicit_dim=False):
+    """{vecN pos; int payload; int splitDim} as in
+    testing/testBuildersSameResult.cu:188-193 (splitDim used only when
+    explicit_dim)."""
+    return Layout(d, scalar_type, 4 * d + 8, 0, 4 * d + 4 if explicit_dim else -1, 4)
+
+
+class MemResource(ctypes.Structure):
+    """cukd_mem_resource: user allocation callbacks (GpuMemoryResource)."""
+    _fields_ = [("ctx", ctypes.c_void_p),
+                ("malloc_fn", ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p,
+                                               ctypes.POINTER(ctypes.c_void_p),
+                                               ctypes.c_size_t, ctypes.c_void_p)),
+                ("free_fn", ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p))]
+
+
+class SpatialNode(ctypes.Structure):
+    _fields_ = [("pos", ctypes.c_float), ("offset", ctypes.c_uint32),
+                ("count", ctypes.c_uint16), ("dim", ctypes.c_int16)]
+
+
+class SpatialTree(ctypes.Structure):
+    """cukd_spatial_tree (spatial-kdtree.h:69-98)."""
+    _fields_ = [("bounds", ctypes.c_float * 16), ("nodes", ctypes.c_void_p),
+                ("prim_ids", ctypes.c_void_p), ("data", ctypes.c_void_p),
+                ("num_prims", ctypes.c_int32), ("num_nodes", ctypes.c_int32)]
+
+
+def lib():
+    """Loads libcukd_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError("libcukd_amd.so is not built: run `python -m cudakdtree_amd.build`")
+    L = ctypes.CDLL(_LIB_PATH)
+    P, i32, f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float
+    LP = ctypes.POINTER(Layout)
+    L.cukd_build_tree.argtypes = [P, i32, LP, ctypes.c_int, P, P, P]
+    L.cukd_build_tree_temp_bytes.argtypes = [i32, LP, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_size_t)]
+    L.cukd_compute_bounds.argtypes = [P, P, i32, LP, P]
+    L.cukd_fcp.argtypes = [ctypes.c_int, P, i32, LP, P, P, i32, f32, f32, P, P,
+                           ctypes.c_int, P, P]
+    L.cukd_knn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, i32, LP, P, P,
+                           i32, f32, P, P, P, ctypes.c_int, P, P]
+    SP = ctypes.POINTER(SpatialTree)
+    L.cukd_spatial_build.argtypes = [SP, P, i32, LP, ctypes.c_int, P, P]
+    L.cukd_spatial_free.argtypes = [SP, P, P]
+    L.cukd_spatial_fcp.argtypes = [ctypes.c_int, SP, LP, P, i32, f32, P, P, P]
+    L.cukd_spatial_knn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, SP, LP, P,
+                                   i32, f32, P, P, P, P]
+    L.cukd_set_traversal_stats.argtypes = [P]
+    L.cukd_last_error.restype = ctypes.c_char_p
+    L.cukd_last_error.argtypes = []
+    L.cukd_version.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != OK:
+        raise CukdError(rc, lib().cukd_last_error().decode())
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _rows(points, layout):
+    """Number of elements in a device tensor of layout.stride-byte rows."""
+    nbytes = points.numel() * points.element_size()
+    if nbytes % layout.stride:
+        raise ValueError("tensor size is not a multiple of the element stride")
+    if not points.is_contiguous():
+        raise ValueError("points must be contiguous")
+    return nbytes // layout.stride
+
+
+# ----------------------------------------------------------------------------
+# builders (builder.h:108-151)
+# ----------------------------------------------------------------------------
+
+def build_tree(points, layout, builder=BUILDER_DEFAULT, world_bounds=None,
+               stream=None, n=None):
+    """Builds the left-balanced tree in place over ``points``."""
+    n = _rows(points, layout) if n is None else n
+    _check(lib().cukd_build_tree(_ptr(points), n, ctypes.byref(layout), builder,
+                                 _ptr(world_bounds), _stream(stream), None))
+    return points
+
+
+def buildTree(points, layout, world_bounds=None, stream=None):
+    return build_tree(points, layout, BUILDER_DEFAULT, world_bounds, stream)
+
+
+def buildTree_thrust(points, layout, world_bounds=None, stream=None):
+    return build_tree(points, layout, BUILDER_THRUST, world_bounds, stream)
+
+
+def buildTree_bitonic(points, layout, world_bounds=None, stream=None):
+    return build_tree(points, layout, BUILDER_BITONIC, world_bounds, stream)
+
+
+def buildTree_inPlace(points, layout, world_bounds=None, stream=None):
+    return build_tree(points, layout, BUILDER_INPLACE, world_bounds, stream)
+
+
+buildTree_inplace = buildTree_inPlace
+
+
+def build_temp_bytes(n, layout, builder=BUILDER_DEFAULT):
+    out = ctypes.c_size_t(0)
+    _check(lib().cukd_build_tree_temp_bytes(n, ctypes.byref(layout), builder,
+                                            ctypes.byref(out)))
+    return out.value
+
+
+def bounds_tensor(layout, device="cuda"):
+    """A device buffer for box_t<point_t>: lower[D] then upper[D]."""
+    import torch
+    dt = torch.float32 if layout.scalar_type == SCALAR_F32 else torch.int32
+    return torch.empty(2 * layout.num_dims, dtype=dt, device=device)
+
+
+def computeBounds(bounds, points, layout, stream=None):
+    n = _rows(points, layout)
+    _check(lib().cukd_compute_bounds(_ptr(bounds), _ptr(points), n, ctypes.byref(layout),
+                                     _stream(stream)))
+    return bounds
+
+
+# ----------------------------------------------------------------------------
+# queries (fcp.h, knn.h)
+# ----------------------------------------------------------------------------
+
+def fcp(tree, layout, queries, traversal=STACK_BASED, cutoff=math.inf, eps=0.0,
+        world_bounds=None, flags=0, stream=None, out=None):
+    """Batched FCP; returns (ids int32[nq], dist2 float32[nq])."""
+    import torch
+    n = _rows(tree, layout)
+    nq = queries.shape[0]
+    if out is None:
+        ids = torch.empty(nq, dtype=torch.int32, device=queries.device)
+        d2 = torch.empty(nq, dtype=torch.float32, device=queries.device)
+    else:
+        ids, d2 = out
+    _check(lib().cukd_fcp(traversal, _ptr(tree), n, ctypes.byref(layout), _ptr(world_bounds),
+                          _ptr(queries), nq, cutoff, eps, _ptr(ids), _ptr(d2), flags,
+                          _stream(stream), None))
+    return ids, d2
+
+
+def knn(tree, layout, queries, k, traversal=STACK_BASED, list_kind=LIST_FIXED,
+        cutoff=math.inf, world_bounds=None, flags=0, stream=None, out=None):
+    """Batched kNN; returns (ids int32[nq,k], dist2 float32[nq,k],
+    max_radius2 float32[nq])."""
+    import torch
+    n = _rows(tree, layout)
+    nq = queries.shape[0]
+    if out is None:
+        ids = torch.empty((nq, k), dtype=torch.int32, device=queries.device)
+        d2 = torch.empty((nq, k), dtype=torch.float32, device=queries.device)
+        ret = torch.empty(nq, dtype=torch.float32, device=queries.device)
+    else:
+        ids, d2, ret = out
+    _check(lib().cukd_knn(traversal, list_kind, k, _ptr(tree), n, ctypes.byref(layout),
+                          _ptr(world_bounds), _ptr(queries), nq, cutoff, _ptr(ids),
+                          _ptr(d2), _ptr(ret), flags, _stream(stream), None))
+    return ids, d2, ret
+
+
+def set_traversal_stats(counter):
+    """counter: a device int64 tensor of one element, or None."""
+    _check(lib().cukd_set_traversal_stats(_ptr(counter)))
+
+
+def _ns(traversal):
+    def _fcp(tree, layout, queries, cutoff=math.inf, eps=0.0, world_bounds=None, **kw):
+        return fcp(tree, layout, queries, traversal, cutoff, eps, world_bounds, **kw)
+
+    def _knn(tree, layout, queries, k, list_kind=LIST_FIXED, cutoff=math.inf,
+             world_bounds=None, **kw):
+        return knn(tree, layout, queries, k, traversal, list_kind, cutoff, world_bounds, **kw)
+    return types.SimpleNamespace(fcp=_fcp, knn=_knn)
+
+
+stackBased = _ns(STACK_BASED)
+stackFree = _ns(STACK_FREE)
+cct = _ns(CCT)

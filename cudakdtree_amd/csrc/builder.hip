@@ -1,0 +1,805 @@
+// builder.hip -- left-balanced k-d tree builder for gfx950.
+//
+// Produces exactly the array buildTree_host produces (builder_thrust.h:
+// 514-601).  The host builder stable-sorts (tag, coord[dim]) once per
+// level, so within a level-L subtree the elements end up ordered by the
+// composite key
+//     (c[d_L], c[d_{L-1}], ..., c[d_0], input index)
+// where d_j is the split dim of the ancestor on level j (repeated dims
+// dropped: they never decide a comparison).  The pivot of a subtree is the
+// element whose rank under that key equals the left-subtree size
+// (ArrayLayoutInStep::pivotPosOf, helpers.h:235-241).  The tree depends
+// only on those selections, so the GPU builder never materialises the
+// host's intermediate orders:
+//
+//  * top levels (subtrees larger than one LDS tile): a per-level,
+//    all-subtrees-at-once radix select over the composite key (3 digit
+//    passes per 32-bit word, a word only when an earlier one ties),
+//    followed by an unordered partition into the next level's compact
+//    layout, pivots written straight to their final node slot;
+//  * bottom levels: one workgroup per subtree loads it into LDS, puts it
+//    in composite-key order once (LSD bitonic passes), then runs every
+//    remaining level as a bitonic sort of (coord, position) keys inside
+//    power-of-two aligned blocks -- the position tie-break makes each
+//    level's sort stable, which reproduces the host's tie chain -- and
+//    writes each pivot to its node slot.
+//
+// All launches are stream-ordered, with no host synchronisation.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "cukd_common.h"
+
+namespace cukd_amd {
+
+enum { SEL_NT = 256, SEL_ITEMS = 8, DIGIT_MAX_BINS = 2048 };
+enum { MAX_WORDS = CUKD_MAX_DIMS + 1 };
+
+// Per-subtree selection state of one level.
+struct SegState {
+  uint32_t krem;      // rank of the pivot among the remaining candidates
+  uint32_t word;      // word being resolved (or resolved word if resolved)
+  uint32_t prefix;    // high bits of 'word' fixed so far
+  uint32_t resolved;  // words[0..word] identify the pivot uniquely
+  uint32_t tiles_done;
+  uint32_t dim;       // split dim of this subtree
+  uint32_t chain;     // split dims, most recent first, 4 bits each
+  uint32_t chain_len; // number of coordinate words in the key
+  uint32_t fixed[MAX_WORDS];
+  uint32_t box[2 * CUKD_MAX_DIMS]; // domain (explicit dims), raw scalar bits
+};
+
+__device__ inline uint32_t chain_at(uint32_t chain, int j) { return (chain >> (4 * j)) & 15u; }
+
+// widestDimension on raw scalar bits (box.h:110-125): ties pick the later dim.
+__device__ inline int widest_dim_bits(const uint32_t *box, int D, bool is_float)
+{
+  int best = 0;
+  if (is_float) {
+    float wb = 0.f;
+    for (int d = 0; d < D; d++) {
+      const float w = __uint_as_float(box[D + d]) - __uint_as_float(box[d]);
+      if (w < wb) continue;
+      wb = w;
+      best = d;
+    }
+  } else {
+    long long wb = 0;
+    for (int d = 0; d < D; d++) {
+      const long long w = (long long)(int)box[D + d] - (long long)(int)box[d];
+      if (w < wb) continue;
+      wb = w;
+      best = d;
+    }
+  }
+  return best;
+}
+
+// chain of a child: its own dim first, then the parent's chain without it
+__device__ inline void child_chain(uint32_t parent_chain, uint32_t parent_len,
+                                   uint32_t dim, uint32_t &chain, uint32_t &len)
+{
+  chain = dim;
+  len = 1;
+  for (uint32_t j = 0; j < parent_len; j++) {
+    const uint32_t d = chain_at(parent_chain, (int)j);
+    if (d == dim) continue;
+    chain |= d << (4 * len);
+    len++;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// zero fill (used instead of hipMemsetAsync so the sequence can be captured)
+// ---------------------------------------------------------------------------
+__global__ void k_zero(uint32_t *p, size_t n)
+{
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// world bounds (builder_common.h:76-163): block min/max of order-preserving
+// keys, one atomic per dimension and block.
+// ---------------------------------------------------------------------------
+__global__ void k_bounds_init(uint32_t *acc, int D)
+{
+  const int d = threadIdx.x;
+  if (d < D) { acc[d] = 0xffffffffu; acc[D + d] = 0u; }
+}
+
+__global__ __launch_bounds__(256) void k_bounds(uint32_t *acc,
+                                                const uint8_t *points, int n,
+                                                cukd_layout L)
+{
+  __shared__ uint32_t s_lo[CUKD_MAX_DIMS][256], s_hi[CUKD_MAX_DIMS][256];
+  const int D = L.num_dims;
+  const bool isf = L.scalar_type == CUKD_SCALAR_F32;
+  uint32_t lo[CUKD_MAX_DIMS], hi[CUKD_MAX_DIMS];
+  for (int d = 0; d < D; d++) { lo[d] = 0xffffffffu; hi[d] = 0u; }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint8_t *e = points + (size_t)i * L.stride;
+    for (int d = 0; d < D; d++) {
+      const uint32_t k = ordered_key(load_coord_bits(e, L, d), isf);
+      lo[d] = min(lo[d], k);
+      hi[d] = max(hi[d], k);
+    }
+  }
+  for (int d = 0; d < D; d++) { s_lo[d][threadIdx.x] = lo[d]; s_hi[d][threadIdx.x] = hi[d]; }
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off)
+      for (int d = 0; d < D; d++) {
+        s_lo[d][threadIdx.x] = min(s_lo[d][threadIdx.x], s_lo[d][threadIdx.x + off]);
+        s_hi[d][threadIdx.x] = max(s_hi[d][threadIdx.x], s_hi[d][threadIdx.x + off]);
+      }
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < D) {
+    atomicMin(&acc[threadIdx.x], s_lo[threadIdx.x][0]);
+    atomicMax(&acc[D + threadIdx.x], s_hi[threadIdx.x][0]);
+  }
+}
+
+// Decodes the accumulated bounds into the caller's box and seeds the root
+// subtree's state (domain = world box, dim = widest, builder_thrust.h:
+// 144-156).
+__global__ void k_bounds_final(const uint32_t *acc, uint32_t *user_box,
+                               SegState *root, cukd_layout L, int explicit_dim)
+{
+  if (threadIdx.x != 0) return;
+  const int D = L.num_dims;
+  const bool isf = L.scalar_type == CUKD_SCALAR_F32;
+  uint32_t box[2 * CUKD_MAX_DIMS];
+  for (int i = 0; i < 2 * D; i++) box[i] = unordered_key(acc[i], isf);
+  if (user_box)
+    for (int i = 0; i < 2 * D; i++) user_box[i] = box[i];
+  if (root) {
+    for (int i = 0; i < 2 * D; i++) root->box[i] = box[i];
+    const int d0 = explicit_dim ? widest_dim_bits(box, D, isf) : 0;
+    root->dim = (uint32_t)d0;
+    root->chain = (uint32_t)d0;
+    root->chain_len = 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// copy-in: working buffer A gets the points, their input index, and the
+// level-0 key.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_copy_in(uint8_t *data, uint32_t *idx,
+                                                 uint32_t *key,
+                                                 const uint8_t *points, int n,
+                                                 cukd_layout L,
+                                                 const SegState *root,
+                                                 int explicit_dim)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int d0 = explicit_dim ? (int)root->dim : 0;
+  const uint8_t *src = points + (size_t)i * L.stride;
+  const int W = L.stride >> 2;
+  uint32_t *dst = (uint32_t *)(data + (size_t)i * L.stride);
+  for (int w = 0; w < W; w++) dst[w] = ((const uint32_t *)src)[w];
+  idx[i] = (uint32_t)i;
+  if (key) key[i] = ordered_key(load_coord_bits(src, L, d0), L.scalar_type == CUKD_SCALAR_F32);
+}
+
+// ---------------------------------------------------------------------------
+// global levels
+// ---------------------------------------------------------------------------
+
+// Per-level reset of the subtree states; round-robin dims and chains are
+// closed forms of the level.
+__global__ void k_level_init(SegState *st, uint32_t *cursors, uint32_t *need,
+                             int n, int level, int D, int explicit_dim)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0)
+    for (int w = 0; w < MAX_WORDS; w++) need[w] = 0u;
+  if (s >= (1 << level)) return;
+  const int node = (1 << level) - 1 + s;
+  SegState &t = st[s];
+  t.krem = (uint32_t)subtree_size(2 * node + 1, n);
+  t.word = 0;
+  t.prefix = 0;
+  t.resolved = 0;
+  t.tiles_done = 0;
+  if (!explicit_dim) {
+    t.dim = (uint32_t)(level % D);
+    uint32_t chain = 0, len = 0;
+    for (int j = level; j >= 0 && (int)len < D; j--) chain |= (uint32_t)(j % D) << (4 * len++);
+    t.chain = chain;
+    t.chain_len = len;
+  }
+  cursors[2 * s] = 0u;
+  cursors[2 * s + 1] = 0u;
+}
+
+// Value of composite-key word w of an element (word 0 is the cached key).
+__device__ inline uint32_t key_word(int w, const SegState &t, uint32_t key0,
+                                    const uint8_t *elem, uint32_t idx,
+                                    const cukd_layout &L, bool isf)
+{
+  if (w == 0) return key0;
+  if (w < (int)t.chain_len)
+    return ordered_key(load_coord_bits(elem, L, (int)chain_at(t.chain, w)), isf);
+  return idx;
+}
+
+// Tiles of size T over the compact layout of one level.  A tile spans at
+// most two subtrees because every subtree on a global level holds at least
+// T elements.
+__device__ inline int tiles_of(const LevelGeom &g, int s, int T)
+{
+  const int a = g.offset(s);
+  const int b = a + g.size(s) - 1;
+  return b / T - a / T + 1;
+}
+
+// One digit pass of the radix select.  Every tile histograms the digit of
+// its candidates; the last tile of each subtree picks the digit bin that
+// holds the pivot rank.
+__global__ __launch_bounds__(SEL_NT) void k_select_pass(
+    LevelGeom g, int word, int shift, int bits, int T,
+    const uint32_t *key, const uint8_t *data, const uint32_t *idx,
+    SegState *st, SegState *child_st, uint32_t *hist, uint32_t *need,
+    cukd_layout L, int explicit_dim)
+{
+  if (word > 0 && __hip_atomic_load(&need[word], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+    return;
+  __shared__ uint32_t s_hist[2][DIGIT_MAX_BINS];
+  __shared__ SegState s_st[2];
+  __shared__ int s_seg[2];
+  __shared__ int s_last[2];
+  const bool isf = L.scalar_type == CUKD_SCALAR_F32;
+  const int nbins = 1 << bits;
+  const int t0 = blockIdx.x * T;
+  const int t1 = min(t0 + T, g.n_elems);
+  const int seg0 = g.seg_of(t0);
+  const int seg1 = g.seg_of(t1 - 1);
+  for (int i = threadIdx.x; i < 2 * nbins; i += blockDim.x) s_hist[i / nbins][i % nbins] = 0u;
+  if (threadIdx.x == 0) { s_seg[0] = seg0; s_seg[1] = seg1; s_st[0] = st[seg0]; }
+  if (threadIdx.x == 1) s_st[1] = st[seg1];
+  __syncthreads();
+  // tiles inside one subtree use slot 0 only
+  const int seg1_begin = seg1 != seg0 ? g.offset(seg1) : g.n_elems;
+  const uint32_t mask = (uint32_t)nbins - 1u;
+  const int hi_shift = shift + bits;
+  for (int p = t0 + threadIdx.x; p < t1; p += blockDim.x) {
+    const int slot = p >= seg1_begin ? 1 : 0;
+    const SegState &t = s_st[slot];
+    if (t.resolved || (int)t.word != word) continue;
+    const uint8_t *elem = data + (size_t)p * L.stride;
+    const uint32_t id = idx[p];
+    const uint32_t k0 = key[p];
+    bool cand = true;
+    for (int w = 0; w < word && cand; w++)
+      cand = key_word(w, t, k0, elem, id, L, isf) == t.fixed[w];
+    if (!cand) continue;
+    const uint32_t v = key_word(word, t, k0, elem, id, L, isf);
+    if (hi_shift < 32 && (v >> hi_shift) != t.prefix) continue;
+    atomicAdd(&s_hist[slot][(v >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  const int nslots = seg1 != seg0 ? 2 : 1;
+  for (int sl = 0; sl < nslots; sl++) {
+    const int s = s_seg[sl];
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
+      const uint32_t c = s_hist[sl][b];
+      if (c) atomicAdd(&hist[(size_t)s * DIGIT_MAX_BINS + b], c);
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int sl = 0; sl < 2; sl++) {
+      s_last[sl] = 0;
+      if (sl < nslots) {
+        const int s = s_seg[sl];
+        const uint32_t done = atomicAdd(&st[s].tiles_done, 1u);
+        s_last[sl] = (int)done == tiles_of(g, s, T) - 1;
+      }
+    }
+  }
+  __syncthreads();
+  for (int sl = 0; sl < nslots; sl++) {
+    if (!s_last[sl]) continue;
+    __threadfence();
+    const int s = s_seg[sl];
+    uint32_t *h = hist + (size_t)s * DIGIT_MAX_BINS;
+    // reuse s_hist[sl] for the subtree's global histogram
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
+      s_hist[sl][b] = __hip_atomic_load(&h[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      h[b] = 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      SegState t = s_st[sl];
+      t.tiles_done = 0;
+      if (!t.resolved && (int)t.word == word) {
+        uint32_t before = 0, b = 0, cnt = 0;
+        for (b = 0; b < (uint32_t)nbins; b++) {
+          cnt = s_hist[sl][b];
+          if (before + cnt > t.krem) break;
+          before += cnt;
+        }
+        t.krem -= before;
+        t.prefix = (hi_shift < 32 ? (t.prefix << bits) : 0u) | b;
+        if (shift == 0) {
+          t.fixed[word] = t.prefix;
+          t.prefix = 0;
+          if (cnt == 1u) {
+            t.resolved = 1;
+          } else if (word + 1 < MAX_WORDS) {
+            t.word = (uint32_t)(word + 1);
+            __hip_atomic_store(&need[word + 1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          if (word == 0 && explicit_dim) {
+            // child domains and dims (updateTagAndSetDim,
+            // builder_thrust.h:259-314): the pivot coordinate clips the
+            // parent box on the split dim.
+            const int D = L.num_dims;
+            const uint32_t pc = unordered_key(t.fixed[0], isf);
+            for (int side = 0; side < 2; side++) {
+              SegState &c = child_st[2 * s + side];
+              uint32_t box[2 * CUKD_MAX_DIMS];
+              for (int i = 0; i < 2 * D; i++) box[i] = t.box[i];
+              box[(side == 0 ? D : 0) + (int)t.dim] = pc;
+              const int cd = widest_dim_bits(box, D, isf);
+              for (int i = 0; i < 2 * D; i++) c.box[i] = box[i];
+              c.dim = (uint32_t)cd;
+              child_chain(t.chain, t.chain_len, (uint32_t)cd, c.chain, c.chain_len);
+            }
+          }
+        }
+      }
+      st[s] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// Partition of every subtree around its pivot: pivots go to their node
+// slot in the output, the rest to the two child subtrees of the next
+// level's compact layout (order inside a child is irrelevant: the next
+// level selects by explicit key).
+__global__ __launch_bounds__(SEL_NT) void k_partition(
+    LevelGeom g, LevelGeom gn, int T, const uint32_t *key,
+    const uint8_t *data, const uint32_t *idx, const SegState *st,
+    const SegState *child_st, uint8_t *ndata, uint32_t *nidx, uint32_t *nkey,
+    uint32_t *cursors, uint8_t *out, cukd_layout L, int explicit_dim,
+    int write_next_key)
+{
+  __shared__ SegState s_st[2];
+  __shared__ uint32_t s_cdim[4];
+  __shared__ uint64_t s_scan[SEL_NT];
+  __shared__ uint32_t s_base[4];
+  const bool isf = L.scalar_type == CUKD_SCALAR_F32;
+  const int D = L.num_dims;
+  const int t0 = blockIdx.x * T;
+  const int t1 = min(t0 + T, g.n_elems);
+  const int seg0 = g.seg_of(t0);
+  const int seg1 = g.seg_of(t1 - 1);
+  if (threadIdx.x == 0) s_st[0] = st[seg0];
+  if (threadIdx.x == 1) s_st[1] = st[seg1];
+  if (threadIdx.x < 4) {
+    const int sl = threadIdx.x >> 1, side = threadIdx.x & 1;
+    const int s = sl ? seg1 : seg0;
+    s_cdim[threadIdx.x] = explicit_dim ? child_st[2 * s + side].dim
+                                       : (uint32_t)((g.L + 1) % D);
+  }
+  __syncthreads();
+  // tiles inside one subtree use slot 0 only
+  const int seg1_begin = seg1 != seg0 ? g.offset(seg1) : g.n_elems;
+  // classify
+  uint8_t cls[SEL_ITEMS]; // bit0-1: side (0 left, 1 right, 2 pivot), bit2: slot
+  uint64_t my = 0;        // 4 x 16-bit counters: (slot, side)
+  for (int k = 0; k < SEL_ITEMS; k++) {
+    const int p = t0 + k * SEL_NT + threadIdx.x;
+    cls[k] = 0xff;
+    if (p >= t1) continue;
+    const int slot = p >= seg1_begin ? 1 : 0;
+    const SegState &t = s_st[slot];
+    const uint8_t *elem = data + (size_t)p * L.stride;
+    const uint32_t id = idx[p];
+    const uint32_t k0 = key[p];
+    int side = 2;
+    for (int w = 0; w <= (int)t.word; w++) {
+      const uint32_t v = key_word(w, t, k0, elem, id, L, isf);
+      if (v != t.fixed[w]) { side = v < t.fixed[w] ? 0 : 1; break; }
+    }
+    cls[k] = (uint8_t)(side | (slot << 2));
+    if (side < 2) my += 1ull << (16 * (slot * 2 + side));
+  }
+  // block exclusive scan of the packed counters
+  s_scan[threadIdx.x] = my;
+  __syncthreads();
+  for (int off = 1; off < SEL_NT; off <<= 1) {
+    const uint64_t v = threadIdx.x >= (unsigned)off ? s_scan[threadIdx.x - off] : 0ull;
+    __syncthreads();
+    s_scan[threadIdx.x] += v;
+    __syncthreads();
+  }
+  const uint64_t incl = s_scan[threadIdx.x];
+  uint64_t excl = incl - my;
+  if (threadIdx.x == SEL_NT - 1) {
+    for (int c = 0; c < 4; c++) {
+      const uint32_t total = (uint32_t)((incl >> (16 * c)) & 0xffffu);
+      const int s = (c >> 1) ? seg1 : seg0;
+      s_base[c] = total ? atomicAdd(&cursors[2 * s + (c & 1)], total) : 0u;
+    }
+  }
+  __syncthreads();
+  for (int k = 0; k < SEL_ITEMS; k++) {
+    if (cls[k] == 0xff) continue;
+    const int p = t0 + k * SEL_NT + threadIdx.x;
+    const int side = cls[k] & 3, slot = cls[k] >> 2;
+    const int s = slot ? seg1 : seg0;
+    const uint8_t *elem = data + (size_t)p * L.stride;
+    if (side == 2) {
+      const int node = (1 << g.L) - 1 + s;
+      copy_elem(out + (size_t)node * L.stride, elem, L, (int)s_st[slot].dim);
+      continue;
+    }
+    const int c = slot * 2 + side;
+    const uint32_t rank = (uint32_t)((excl >> (16 * c)) & 0xffffu);
+    excl += 1ull << (16 * c);
+    const int dst = gn.offset(2 * s + side) + (int)(s_base[c] + rank);
+    const int W = L.stride >> 2;
+    uint32_t *dw = (uint32_t *)(ndata + (size_t)dst * L.stride);
+    for (int w = 0; w < W; w++) dw[w] = ((const uint32_t *)elem)[w];
+    nidx[dst] = idx[p];
+    if (write_next_key)
+      nkey[dst] = ordered_key(load_coord_bits(elem, L, (int)s_cdim[c]), isf);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS finish: one workgroup builds a whole subtree of at most CAP-1 nodes.
+// ---------------------------------------------------------------------------
+
+// Bitonic sort of keys[0..n) as independent ascending blocks of size B
+// (n and B powers of two).
+template <int NT>
+__device__ inline void bitonic_blocks(uint64_t *keys, int n, int B)
+{
+  for (int k = 2; k <= B; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < (n >> 1); i += NT) {
+        const int lo = 2 * i - (i & (j - 1));
+        const int hi = lo + j;
+        const bool asc = (k == B) || ((lo & k) == 0);
+        const uint64_t a = keys[lo], b = keys[hi];
+        if ((a > b) == asc) { keys[lo] = b; keys[hi] = a; }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <int CAP, int NT, bool EXPLICIT>
+__global__ __launch_bounds__(NT) void k_finish(
+    int n, int L0, const uint8_t *data, const uint32_t *idx,
+    const SegState *st, uint8_t *out, cukd_layout L)
+{
+  __shared__ uint64_t s_key[CAP];
+  __shared__ uint16_t s_slot[2][CAP];
+  __shared__ uint32_t s_split[EXPLICIT ? CAP : 1];
+  __shared__ uint8_t s_sdim[EXPLICIT ? CAP : 1];
+  __shared__ uint8_t s_bdim[EXPLICIT ? CAP / 2 : 1];
+  __shared__ uint32_t s_box[2 * CUKD_MAX_DIMS];
+
+  const bool isf = L.scalar_type == CUKD_SCALAR_F32;
+  const int D = L.num_dims;
+  const int h = num_levels_for(n);
+  const int hs = h - L0;            // levels in this subtree
+  const int Cs = 1 << hs;           // aligned capacity
+  const int s = blockIdx.x;
+  LevelGeom g;
+  g.init(n, L0);
+  const int g_node = (1 << L0) - 1 + s;
+  const int m = g.size(s);
+  const int base = L0 > 0 ? g.offset(s) : 0;
+  const uint8_t *sdata = data + (size_t)base * L.stride;
+  const uint32_t *sidx = idx + base;
+  const SegState &gst = st[s];
+  if (EXPLICIT && threadIdx.x < 2 * D) s_box[threadIdx.x] = gst.box[threadIdx.x];
+
+  int cur = 0;
+  for (int p = threadIdx.x; p < Cs; p += NT) s_slot[0][p] = (uint16_t)p;
+  __syncthreads();
+
+  // Entry order.  The host builder's order inside this subtree before its
+  // level-L0 sort is the composite key of the levels above: the subtree's
+  // own chain without its first dim, then the input index.  Establish it
+  // with LSD passes: input index first, then the chain dims from least to
+  // most significant, each tie-broken by the current position.
+  if (L0 > 0) {
+    uint32_t chain = 0, len = 0;
+    if (EXPLICIT) {
+      chain = gst.chain;
+      len = gst.chain_len;
+    } else {
+      for (int l = L0; l >= 0 && (int)len < D; l--) chain |= (uint32_t)(l % D) << (4 * len++);
+    }
+    for (int j = (int)len; j >= 1; j--) {
+      const int d = j < (int)len ? (int)chain_at(chain, j) : -1;
+      for (int p = threadIdx.x; p < Cs; p += NT) {
+        uint64_t k = ~0ull;
+        if (p < m) {
+          const int sl = s_slot[cur][p];
+          const uint64_t hi = d < 0 ? (uint64_t)sidx[sl]
+              : (uint64_t)ordered_key(load_coord_bits(sdata + (size_t)sl * L.stride, L, d), isf);
+          k = (hi << 32) | (uint64_t)p;
+        }
+        s_key[p] = k;
+      }
+      __syncthreads();
+      bitonic_blocks<NT>(s_key, Cs, Cs);
+      for (int q = threadIdx.x; q < m; q += NT)
+        s_slot[cur ^ 1][q] = s_slot[cur][(int)(uint32_t)s_key[q]];
+      __syncthreads();
+      cur ^= 1;
+    }
+  }
+
+  for (int j = 0; j < hs; j++) {
+    const int B = Cs >> j;
+    const int nblk = 1 << j;
+    const int first_node = (g_node + 1) * nblk - 1;
+    // dims of this depth's blocks
+    if (EXPLICIT) {
+      for (int b = threadIdx.x; b < nblk; b += NT) {
+        if (first_node + b >= n) { s_bdim[b] = 0; continue; }  // no such node
+        uint32_t box[2 * CUKD_MAX_DIMS];
+        for (int i = 0; i < 2 * D; i++) box[i] = s_box[i];
+        int local = nblk - 1 + b;
+        while (local > 0) {
+          const int parent = (local - 1) >> 1;
+          const int pd = s_sdim[parent];
+          const uint32_t sv = s_split[parent];
+          if (local & 1) {
+            const uint32_t u = box[D + pd];
+            const bool lt = isf ? (__uint_as_float(u) < __uint_as_float(sv)) : ((int)u < (int)sv);
+            box[D + pd] = lt ? u : sv;
+          } else {
+            const uint32_t l = box[pd];
+            const bool lt = isf ? (__uint_as_float(sv) < __uint_as_float(l)) : ((int)sv < (int)l);
+            box[pd] = lt ? l : sv;
+          }
+          local = parent;
+        }
+        s_bdim[b] = (uint8_t)widest_dim_bits(box, D, isf);
+      }
+      __syncthreads();
+    }
+    const int rr_dim = (L0 + j) % D;
+    for (int p = threadIdx.x; p < Cs; p += NT) {
+      const int b = p / B;
+      const int node = first_node + b;
+      const int sz = subtree_size(node, n);
+      uint64_t k = ~0ull;
+      if (p - b * B < sz) {
+        const int d = EXPLICIT ? (int)s_bdim[b] : rr_dim;
+        const int sl = s_slot[cur][p];
+        k = ((uint64_t)ordered_key(load_coord_bits(sdata + (size_t)sl * L.stride, L, d), isf) << 32) | (uint64_t)p;
+      }
+      s_key[p] = k;
+    }
+    __syncthreads();
+    if (B > 1) bitonic_blocks<NT>(s_key, Cs, B);
+    for (int q = threadIdx.x; q < Cs; q += NT) {
+      const uint64_t k = s_key[q];
+      if (k == ~0ull) continue;
+      const int b = q / B;
+      const int node = first_node + b;
+      const int r = q - b * B;
+      const int left = subtree_size(2 * node + 1, n);
+      const int sl = s_slot[cur][(int)(uint32_t)k];
+      if (r == left) {
+        const int d = EXPLICIT ? (int)s_bdim[b] : rr_dim;
+        copy_elem(out + (size_t)node * L.stride, sdata + (size_t)sl * L.stride, L, d);
+        if (EXPLICIT) {
+          s_split[nblk - 1 + b] = load_coord_bits(sdata + (size_t)sl * L.stride, L, d);
+          s_sdim[nblk - 1 + b] = (uint8_t)d;
+        }
+      } else {
+        const int np = r < left ? b * B + r : b * B + (B >> 1) + (r - left - 1);
+        s_slot[cur ^ 1][np] = (uint16_t)sl;
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct BuildPlan {
+  int n, h, L0, cap, T;
+  size_t stride;
+  size_t off_dataA, off_idxA, off_keyA, off_dataB, off_idxB, off_keyB;
+  size_t off_st0, off_st1, off_hist, off_cursors, off_need, off_acc;
+  size_t total;
+};
+
+inline size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+void make_plan(int n, const cukd_layout &L, BuildPlan &P)
+{
+  P.n = n;
+  P.h = num_levels_for(n);
+  P.stride = (size_t)L.stride;
+  int cap = 8192;
+  int L0 = 0;
+  for (cap = 8192; cap >= 512; cap >>= 1) {
+    const int lg = 31 - __builtin_clz((unsigned)cap);
+    L0 = P.h - lg > 0 ? P.h - lg : 0;
+    if ((1 << L0) >= 256 || cap == 512) break;
+  }
+  if (L0 == 0) {
+    // whole tree in one workgroup: capacity = 2^h
+    cap = 512;
+    while (cap < (1 << P.h)) cap <<= 1;
+  }
+  P.L0 = L0;
+  P.cap = cap;
+  P.T = cap / 2 < SEL_NT * SEL_ITEMS ? cap / 2 : SEL_NT * SEL_ITEMS;
+  size_t o = 0;
+  const size_t nn = (size_t)n;
+  P.off_dataA = o; o = align_up(o + nn * P.stride);
+  P.off_idxA = o; o = align_up(o + nn * 4);
+  P.off_keyA = o; o = align_up(o + nn * 4);
+  if (L0 > 0) {
+    P.off_dataB = o; o = align_up(o + nn * P.stride);
+    P.off_idxB = o; o = align_up(o + nn * 4);
+    P.off_keyB = o; o = align_up(o + nn * 4);
+  } else {
+    P.off_dataB = P.off_idxB = P.off_keyB = 0;
+  }
+  const size_t nst = (size_t)1 << L0;
+  P.off_st0 = o; o = align_up(o + nst * sizeof(SegState));
+  P.off_st1 = o; o = align_up(o + nst * sizeof(SegState));
+  const size_t nseg_prev = L0 > 0 ? ((size_t)1 << (L0 - 1)) : 1;
+  P.off_hist = o; o = align_up(o + nseg_prev * DIGIT_MAX_BINS * 4);
+  P.off_cursors = o; o = align_up(o + nseg_prev * 2 * 4);
+  P.off_need = o; o = align_up(o + MAX_WORDS * 4);
+  P.off_acc = o; o = align_up(o + 2 * CUKD_MAX_DIMS * 4);
+  P.total = o;
+}
+
+template <int CAP, int NT>
+hipError_t launch_finish(bool expl, int blocks, hipStream_t s, int n, int L0,
+                         const uint8_t *data, const uint32_t *idx,
+                         const SegState *st, uint8_t *out, const cukd_layout &L)
+{
+  if (expl)
+    hipLaunchKernelGGL((k_finish<CAP, NT, true>), dim3(blocks), dim3(NT), 0, s,
+                       n, L0, data, idx, st, out, L);
+  else
+    hipLaunchKernelGGL((k_finish<CAP, NT, false>), dim3(blocks), dim3(NT), 0, s,
+                       n, L0, data, idx, st, out, L);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// World bounds only (cukd_compute_bounds): the caller's box doubles as the
+// accumulator, so no temporary memory is needed.
+int build_bounds_only(void *d_bounds, const void *d_points, int n, const cukd_layout &L,
+                      hipStream_t s)
+{
+  uint32_t *acc = (uint32_t *)d_bounds;
+  hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, acc, L.num_dims);
+  int blocks = (n + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, s, acc, (const uint8_t *)d_points, n,
+                     L);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, s, (const uint32_t *)acc, acc,
+                     (SegState *)nullptr, L, 0);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_last_error("bounds kernel launch failed: %s", hipGetErrorString(e));
+    return CUKD_ERR_HIP;
+  }
+  return CUKD_OK;
+}
+
+size_t build_temp_bytes(int n, const cukd_layout &L)
+{
+  BuildPlan P;
+  make_plan(n, L, P);
+  return P.total;
+}
+
+int build_tree_impl(void *d_points, int n, const cukd_layout &L,
+                    void *d_bounds, hipStream_t s, const MemResource &mem)
+{
+  const bool expl = L.dim_offset >= 0;
+  const bool isf = L.scalar_type == CUKD_SCALAR_F32;
+  (void)isf;
+  BuildPlan P;
+  make_plan(n, L, P);
+  uint8_t *tmp = nullptr;
+  if (mem.alloc((void **)&tmp, P.total, s) != 0) return CUKD_ERR_ALLOC;
+  uint8_t *dataA = tmp + P.off_dataA, *dataB = tmp + P.off_dataB;
+  uint32_t *idxA = (uint32_t *)(tmp + P.off_idxA), *idxB = (uint32_t *)(tmp + P.off_idxB);
+  uint32_t *keyA = (uint32_t *)(tmp + P.off_keyA), *keyB = (uint32_t *)(tmp + P.off_keyB);
+  SegState *st[2] = {(SegState *)(tmp + P.off_st0), (SegState *)(tmp + P.off_st1)};
+  uint32_t *hist = (uint32_t *)(tmp + P.off_hist);
+  uint32_t *cursors = (uint32_t *)(tmp + P.off_cursors);
+  uint32_t *need = (uint32_t *)(tmp + P.off_need);
+  uint32_t *acc = (uint32_t *)(tmp + P.off_acc);
+  uint8_t *pts = (uint8_t *)d_points;
+  const int D = L.num_dims;
+
+  // zero the small state area (states, histograms, cursors, flags)
+  {
+    const size_t words = (P.off_acc - P.off_st0) / 4;
+    hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, s, (uint32_t *)(tmp + P.off_st0), words);
+  }
+  if (d_bounds || expl) {
+    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, acc, D);
+    int blocks = (n + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, s, acc, (const uint8_t *)pts, n, L);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, s, (const uint32_t *)acc,
+                       (uint32_t *)d_bounds, st[0], L, expl ? 1 : 0);
+  }
+  hipLaunchKernelGGL(k_copy_in, dim3((n + 255) / 256), dim3(256), 0, s, dataA, idxA,
+                     P.L0 > 0 ? keyA : (uint32_t *)nullptr, (const uint8_t *)pts, n, L,
+                     (const SegState *)st[0], expl ? 1 : 0);
+  uint8_t *dcur = dataA, *dnext = dataB;
+  uint32_t *icur = idxA, *inext = idxB, *kcur = keyA, *knext = keyB;
+  for (int level = 0; level < P.L0; level++) {
+    LevelGeom g, gn;
+    g.init(n, level);
+    gn.init(n, level + 1);
+    SegState *sc = st[level & 1], *sn = st[(level + 1) & 1];
+    hipLaunchKernelGGL(k_level_init, dim3(((1 << level) + 255) / 256), dim3(256), 0, s,
+                       sc, cursors, need, n, level, D, expl ? 1 : 0);
+    const int tiles = (g.n_elems + P.T - 1) / P.T;
+    const int max_words = expl ? D + 1 : (level + 1 < D ? level + 1 : D) + 1;
+    for (int w = 0; w < max_words; w++) {
+      const int shifts[3] = {21, 10, 0};
+      const int bitsv[3] = {11, 11, 10};
+      for (int d = 0; d < 3; d++)
+        hipLaunchKernelGGL(k_select_pass, dim3(tiles), dim3(SEL_NT), 0, s, g, w, shifts[d],
+                           bitsv[d], P.T, (const uint32_t *)kcur, (const uint8_t *)dcur,
+                           (const uint32_t *)icur, sc, sn, hist, need, L, expl ? 1 : 0);
+    }
+    hipLaunchKernelGGL(k_partition, dim3(tiles), dim3(SEL_NT), 0, s, g, gn, P.T,
+                       (const uint32_t *)kcur, (const uint8_t *)dcur, (const uint32_t *)icur,
+                       (const SegState *)sc, (const SegState *)sn, dnext, inext, knext, cursors,
+                       pts, L, expl ? 1 : 0, level + 1 < P.L0 ? 1 : 0);
+    uint8_t *t8 = dcur; dcur = dnext; dnext = t8;
+    uint32_t *t32 = icur; icur = inext; inext = t32;
+    t32 = kcur; kcur = knext; knext = t32;
+  }
+  const int blocks = 1 << P.L0;
+  const SegState *sfin = st[P.L0 & 1];
+  hipError_t e = hipSuccess;
+  switch (P.cap) {
+    case 512: e = launch_finish<512, 256>(expl, blocks, s, n, P.L0, dcur, icur, sfin, pts, L); break;
+    case 1024: e = launch_finish<1024, 256>(expl, blocks, s, n, P.L0, dcur, icur, sfin, pts, L); break;
+    case 2048: e = launch_finish<2048, 512>(expl, blocks, s, n, P.L0, dcur, icur, sfin, pts, L); break;
+    case 4096: e = launch_finish<4096, 512>(expl, blocks, s, n, P.L0, dcur, icur, sfin, pts, L); break;
+    default: e = launch_finish<8192, 512>(expl, blocks, s, n, P.L0, dcur, icur, sfin, pts, L); break;
+  }
+  if (e == hipSuccess) e = hipGetLastError();
+  mem.release(tmp, s);
+  if (e != hipSuccess) {
+    set_last_error("builder kernel launch failed: %s", hipGetErrorString(e));
+    return CUKD_ERR_HIP;
+  }
+  return CUKD_OK;
+}
+
+}  // namespace cukd_amd

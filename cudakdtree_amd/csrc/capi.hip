@@ -1,0 +1,329 @@
+// capi.hip -- the extern "C" boundary (include/cukd_amd.h): argument checks,
+// error reporting, the default memory resource, and dispatch to the
+// builder and query kernels.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "cukd_common.h"
+#include "query_impl.h"
+
+namespace cukd_amd {
+
+static thread_local char g_last_error[512] = "";
+static unsigned long long *g_stats = nullptr;
+
+void set_last_error(const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+}
+
+int MemResource::alloc(void **p, size_t bytes, hipStream_t s) const
+{
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  if (user && user->malloc_fn) {
+    if (user->malloc_fn(user->ctx, p, bytes, s) != 0 || !*p) {
+      set_last_error("memory resource failed to allocate %zu bytes", bytes);
+      return -1;
+    }
+    return 0;
+  }
+  const hipError_t e = hipMallocAsync(p, bytes, s);
+  if (e != hipSuccess || !*p) {
+    set_last_error("hipMallocAsync(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+void MemResource::release(void *p, hipStream_t s) const
+{
+  if (!p) return;
+  if (user && user->free_fn) {
+    user->free_fn(user->ctx, p, s);
+    return;
+  }
+  (void)hipFreeAsync(p, s);
+}
+
+int build_tree_impl(void *d_points, int n, const cukd_layout &L, void *d_bounds,
+                    hipStream_t s, const MemResource &mem);
+size_t build_temp_bytes(int n, const cukd_layout &L);
+int build_bounds_only(void *d_bounds, const void *d_points, int n, const cukd_layout &L,
+                      hipStream_t s);
+
+hipError_t launch_query_1(int, const QueryArgs &, hipStream_t);
+hipError_t launch_query_2(int, const QueryArgs &, hipStream_t);
+hipError_t launch_query_3(int, const QueryArgs &, hipStream_t);
+hipError_t launch_query_4(int, const QueryArgs &, hipStream_t);
+hipError_t launch_query_5(int, const QueryArgs &, hipStream_t);
+hipError_t launch_query_6(int, const QueryArgs &, hipStream_t);
+hipError_t launch_query_7(int, const QueryArgs &, hipStream_t);
+hipError_t launch_query_8(int, const QueryArgs &, hipStream_t);
+
+hipError_t launch_query(int D, int method, const QueryArgs &a, hipStream_t s)
+{
+  switch (D) {
+    case 1: return launch_query_1(method, a, s);
+    case 2: return launch_query_2(method, a, s);
+    case 3: return launch_query_3(method, a, s);
+    case 4: return launch_query_4(method, a, s);
+    case 5: return launch_query_5(method, a, s);
+    case 6: return launch_query_6(method, a, s);
+    case 7: return launch_query_7(method, a, s);
+    default: return launch_query_8(method, a, s);
+  }
+}
+
+static int check_layout(const cukd_layout *L)
+{
+  if (!L) { set_last_error("layout is null"); return CUKD_ERR_INVALID; }
+  if (L->num_dims < 1 || L->num_dims > CUKD_MAX_DIMS) {
+    set_last_error("num_dims %d not in 1..%d", L->num_dims, CUKD_MAX_DIMS);
+    return CUKD_ERR_INVALID;
+  }
+  if (L->scalar_type != CUKD_SCALAR_F32 && L->scalar_type != CUKD_SCALAR_I32) {
+    set_last_error("unknown scalar_type %d", L->scalar_type);
+    return CUKD_ERR_INVALID;
+  }
+  if (L->stride < 4 || (L->stride & 3) || L->stride > 256) {
+    set_last_error("stride %d must be a multiple of 4 in [4,256]", L->stride);
+    return CUKD_ERR_INVALID;
+  }
+  if (L->point_offset < 0 || (L->point_offset & 3) ||
+      L->point_offset + 4 * L->num_dims > L->stride) {
+    set_last_error("point_offset %d does not fit %d coordinates in stride %d",
+                   L->point_offset, L->num_dims, L->stride);
+    return CUKD_ERR_INVALID;
+  }
+  if (L->dim_offset >= 0) {
+    if (L->dim_bytes != 1 && L->dim_bytes != 2 && L->dim_bytes != 4) {
+      set_last_error("dim_bytes %d must be 1, 2 or 4", L->dim_bytes);
+      return CUKD_ERR_INVALID;
+    }
+    if (L->dim_offset % L->dim_bytes || L->dim_offset + L->dim_bytes > L->stride) {
+      set_last_error("dim field at %d (%d bytes) misaligned or outside the element",
+                     L->dim_offset, L->dim_bytes);
+      return CUKD_ERR_INVALID;
+    }
+    const int dlo = L->dim_offset, dhi = L->dim_offset + L->dim_bytes;
+    const int plo = L->point_offset, phi = L->point_offset + 4 * L->num_dims;
+    if (dlo < phi && plo < dhi) {
+      set_last_error("dim field overlaps the coordinates");
+      return CUKD_ERR_INVALID;
+    }
+  }
+  return CUKD_OK;
+}
+
+}  // namespace cukd_amd
+
+using namespace cukd_amd;
+
+extern "C" {
+
+const char *cukd_last_error(void) { return g_last_error; }
+
+int cukd_version(void) { return 100; }
+
+int cukd_set_traversal_stats(unsigned long long *d_counter)
+{
+  g_stats = d_counter;
+  return CUKD_OK;
+}
+
+int cukd_build_tree_temp_bytes(int32_t n, const cukd_layout *layout, int builder,
+                               size_t *bytes)
+{
+  g_last_error[0] = 0;
+  if (int rc = check_layout(layout)) return rc;
+  if (!bytes) { set_last_error("bytes is null"); return CUKD_ERR_INVALID; }
+  (void)builder;
+  *bytes = n < 1 ? 0 : build_temp_bytes(n, *layout);
+  return CUKD_OK;
+}
+
+int cukd_build_tree(void *d_points, int32_t n, const cukd_layout *layout, int builder,
+                    void *d_world_bounds, hipStream_t stream, const cukd_mem_resource *mem)
+{
+  g_last_error[0] = 0;
+  if (int rc = check_layout(layout)) return rc;
+  if (builder < CUKD_BUILDER_DEFAULT || builder > CUKD_BUILDER_INPLACE) {
+    set_last_error("unknown builder %d", builder);
+    return CUKD_ERR_INVALID;
+  }
+  if (n < 1) return CUKD_OK;
+  if (n >= (1 << 30)) {
+    set_last_error("n = %d exceeds the 2^30 limit of the 30-entry traversal stack", n);
+    return CUKD_ERR_INVALID;
+  }
+  if (!d_points) { set_last_error("d_points is null"); return CUKD_ERR_INVALID; }
+  if (layout->dim_offset >= 0 && !d_world_bounds) {
+    set_last_error("asked to build k-d tree over nodes with explicit dims, "
+                   "but no memory for world bounds provided");
+    return CUKD_ERR_NO_BOUNDS;
+  }
+  MemResource mr{mem};
+  return build_tree_impl(d_points, n, *layout, d_world_bounds, stream, mr);
+}
+
+int cukd_compute_bounds(void *d_bounds, const void *d_points, int32_t n,
+                        const cukd_layout *layout, hipStream_t stream)
+{
+  g_last_error[0] = 0;
+  if (int rc = check_layout(layout)) return rc;
+  if (!d_bounds || (!d_points && n > 0)) {
+    set_last_error("null pointer");
+    return CUKD_ERR_INVALID;
+  }
+  if (n < 1) {
+    set_last_error("computeBounds of an empty set");
+    return CUKD_ERR_INVALID;
+  }
+  return build_bounds_only(d_bounds, d_points, n, *layout, stream);
+}
+
+static int check_query_common(int traversal, const void *d_tree, int32_t n,
+                              const cukd_layout *layout, const void *d_world_bounds,
+                              const float *d_queries, int32_t nq, int32_t *d_ids)
+{
+  if (int rc = check_layout(layout)) return rc;
+  if (layout->scalar_type != CUKD_SCALAR_F32) {
+    set_last_error("queries need float coordinates (the reference defines no "
+                   "distance math for int points)");
+    return CUKD_ERR_UNSUPPORTED;
+  }
+  if (traversal < CUKD_STACK_BASED || traversal > CUKD_CCT) {
+    set_last_error("unknown traversal %d", traversal);
+    return CUKD_ERR_INVALID;
+  }
+  if (n < 0 || nq < 0) { set_last_error("negative size"); return CUKD_ERR_INVALID; }
+  if (n >= (1 << 30)) { set_last_error("tree too large"); return CUKD_ERR_INVALID; }
+  if (nq > 0 && (!d_queries || !d_ids || (n > 0 && !d_tree))) {
+    set_last_error("null pointer");
+    return CUKD_ERR_INVALID;
+  }
+  if (traversal == CUKD_CCT && !d_world_bounds) {
+    set_last_error("cct traversal needs world bounds");
+    return CUKD_ERR_INVALID;
+  }
+  return CUKD_OK;
+}
+
+static TreeView make_view(const void *d_tree, int32_t n, const cukd_layout *L)
+{
+  TreeView t;
+  t.base = (const uint8_t *)d_tree;
+  t.n = n;
+  t.stride = L->stride;
+  t.point_offset = L->point_offset;
+  t.dim_offset = L->dim_offset;
+  t.dim_bytes = L->dim_bytes;
+  t.num_dims = L->num_dims;
+  return t;
+}
+
+int cukd_fcp(int traversal, const void *d_tree, int32_t n, const cukd_layout *layout,
+             const void *d_world_bounds, const float *d_queries, int32_t nq, float cutoff,
+             float eps, int32_t *d_ids, float *d_dist2, int flags, hipStream_t stream,
+             const cukd_mem_resource *mem)
+{
+  g_last_error[0] = 0;
+  if (int rc = check_query_common(traversal, d_tree, n, layout, d_world_bounds, d_queries,
+                                  nq, d_ids))
+    return rc;
+  if (nq == 0) return CUKD_OK;
+  (void)flags; (void)mem;
+  QueryArgs a;
+  a.t = make_view(d_tree, n, layout);
+  a.queries = d_queries;
+  a.nq = nq;
+  a.world_bounds = (const float *)d_world_bounds;
+  a.cutoff = cutoff;
+  a.eps = eps;
+  a.k = 1;
+  a.list_kind = -1;
+  a.order = nullptr;
+  a.ids = d_ids;
+  a.dist2 = d_dist2;
+  a.ret = nullptr;
+  a.stats = g_stats;
+  const hipError_t e = launch_query(layout->num_dims, traversal, a, stream);
+  if (e != hipSuccess) {
+    set_last_error("fcp launch failed: %s", hipGetErrorString(e));
+    return CUKD_ERR_HIP;
+  }
+  return CUKD_OK;
+}
+
+int cukd_knn(int traversal, int list_kind, int k, const void *d_tree, int32_t n,
+             const cukd_layout *layout, const void *d_world_bounds, const float *d_queries,
+             int32_t nq, float cutoff, int32_t *d_ids, float *d_dist2, float *d_max_radius2,
+             int flags, hipStream_t stream, const cukd_mem_resource *mem)
+{
+  g_last_error[0] = 0;
+  if (int rc = check_query_common(traversal, d_tree, n, layout, d_world_bounds, d_queries,
+                                  nq, d_ids))
+    return rc;
+  if (k < 1 || k > 64) { set_last_error("k = %d not in 1..64", k); return CUKD_ERR_INVALID; }
+  if (list_kind != CUKD_LIST_FIXED && list_kind != CUKD_LIST_HEAP) {
+    set_last_error("unknown list kind %d", list_kind);
+    return CUKD_ERR_INVALID;
+  }
+  if (nq > 0 && !d_dist2) { set_last_error("d_dist2 is null"); return CUKD_ERR_INVALID; }
+  if (nq == 0) return CUKD_OK;
+  (void)flags; (void)mem;
+  QueryArgs a;
+  a.t = make_view(d_tree, n, layout);
+  a.queries = d_queries;
+  a.nq = nq;
+  a.world_bounds = (const float *)d_world_bounds;
+  a.cutoff = cutoff;
+  a.eps = 0.f;
+  a.k = k;
+  a.list_kind = list_kind;
+  a.order = nullptr;
+  a.ids = d_ids;
+  a.dist2 = d_dist2;
+  a.ret = d_max_radius2;
+  a.stats = g_stats;
+  const hipError_t e = launch_query(layout->num_dims, traversal, a, stream);
+  if (e != hipSuccess) {
+    set_last_error("knn launch failed: %s", hipGetErrorString(e));
+    return CUKD_ERR_HIP;
+  }
+  return CUKD_OK;
+}
+
+int cukd_spatial_build(cukd_spatial_tree *, const void *, int32_t, const cukd_layout *, int,
+                       hipStream_t, const cukd_mem_resource *)
+{
+  set_last_error("spatial k-d tree not built into this library version");
+  return CUKD_ERR_UNSUPPORTED;
+}
+
+int cukd_spatial_free(cukd_spatial_tree *, hipStream_t, const cukd_mem_resource *)
+{
+  set_last_error("spatial k-d tree not built into this library version");
+  return CUKD_ERR_UNSUPPORTED;
+}
+
+int cukd_spatial_fcp(int, const cukd_spatial_tree *, const cukd_layout *, const float *,
+                     int32_t, float, int32_t *, float *, hipStream_t)
+{
+  set_last_error("spatial k-d tree not built into this library version");
+  return CUKD_ERR_UNSUPPORTED;
+}
+
+int cukd_spatial_knn(int, int, int, const cukd_spatial_tree *, const cukd_layout *,
+                     const float *, int32_t, float, int32_t *, float *, float *, hipStream_t)
+{
+  set_last_error("spatial k-d tree not built into this library version");
+  return CUKD_ERR_UNSUPPORTED;
+}
+
+}  // extern "C"

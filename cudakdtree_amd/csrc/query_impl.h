@@ -1,0 +1,235 @@
+// query_impl.h -- batched FCP / kNN kernels over a left-balanced k-d tree.
+//
+// One query per lane.  The traversals and result types are the public
+// device-inline ones of include/cukd/ (traverse.h, fcp.h, knn.h), run
+// through an accessor over the run-time element layout, so the batched
+// path and user kernels share one implementation; ids and dist2 bits
+// equal the CPU oracle's, ties included.
+#pragma once
+#pragma clang fp contract(off)
+
+#include <new>
+
+#include "cukd/knn.h"
+#include "cukd_common.h"
+
+namespace cukd_amd {
+
+// node accessor over a cukd_layout-described array
+struct TreeView {
+  const uint8_t *base;
+  int n;
+  int stride;
+  int point_offset;
+  int dim_offset;
+  int dim_bytes;
+  int num_dims;
+
+  __device__ inline int size() const { return n; }
+  template <int D>
+  __device__ inline void point(int i, float (&p)[D]) const
+  {
+    const float *src = (const float *)(base + (size_t)i * stride + point_offset);
+#pragma unroll
+    for (int d = 0; d < D; d++) p[d] = src[d];
+  }
+  __device__ inline int dim(int i) const
+  {
+    if (dim_offset < 0) {
+      const int l = 31 - __clz(i + 1);
+      return (num_dims & (num_dims - 1)) == 0 ? (l & (num_dims - 1)) : l % num_dims;
+    }
+    const uint8_t *p = base + (size_t)i * stride + dim_offset;
+    if (dim_bytes == 1) return *p;
+    if (dim_bytes == 2) return *(const uint16_t *)p;
+    return *(const int32_t *)p;
+  }
+};
+
+// FixedCandidateList with a run-time k (<= 64); entries live in scratch
+struct DynFixedList {
+  uint64_t entry[64];
+  int k;
+  __device__ inline float maxRadius2() const { return __uint_as_float((uint32_t)(entry[k - 1] >> 32)); }
+  __device__ inline float initialCullDist2() const { return maxRadius2(); }
+  __device__ inline float processCandidate(int id, float d2)
+  {
+    uint64_t v = ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)id;
+    for (int i = 0; i < k; i++) {
+      const uint64_t lo = entry[i] < v ? entry[i] : v;
+      const uint64_t hi = entry[i] < v ? v : entry[i];
+      entry[i] = lo;
+      v = hi;
+    }
+    return maxRadius2();
+  }
+};
+
+// ---------------------------------------------------------------------------
+// per-result-type set-up and output
+// ---------------------------------------------------------------------------
+template <class R> struct ResultOps;
+
+template <> struct ResultOps<cukd::FCPResult> {
+  struct State { cukd::FCPResult r; };
+  __device__ static inline cukd::FCPResult &init(State &s, float cutoff, int)
+  {
+    s.r.clear(cutoff * cutoff);
+    return s.r;
+  }
+  __device__ static inline void write(const State &s, int q, int, int32_t *ids, float *dist2, float *)
+  {
+    ids[q] = s.r.closestPrimID;
+    if (dist2) dist2[q] = s.r.closestDist2;
+  }
+};
+
+template <int K> struct ResultOps<cukd::FixedCandidateList<K>> {
+  struct State { cukd::FixedCandidateList<K> r; __device__ State() : r(0.f) {} };
+  __device__ static inline cukd::FixedCandidateList<K> &init(State &s, float cutoff, int)
+  {
+    s.r = cukd::FixedCandidateList<K>(cutoff);
+    return s.r;
+  }
+  __device__ static inline void write(const State &s, int q, int, int32_t *ids, float *dist2, float *ret)
+  {
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+      ids[(size_t)q * K + i] = (int32_t)(uint32_t)s.r.entry[i];
+      dist2[(size_t)q * K + i] = __uint_as_float((uint32_t)(s.r.entry[i] >> 32));
+    }
+    if (ret) ret[q] = s.r.maxRadius2();
+  }
+};
+
+template <> struct ResultOps<DynFixedList> {
+  struct State { DynFixedList r; };
+  __device__ static inline DynFixedList &init(State &s, float cutoff, int k)
+  {
+    s.r.k = k;
+    const uint64_t v = ((uint64_t)__float_as_uint(cutoff * cutoff) << 32) | 0xffffffffu;
+    for (int i = 0; i < k; i++) s.r.entry[i] = v;
+    return s.r;
+  }
+  __device__ static inline void write(const State &s, int q, int k, int32_t *ids, float *dist2, float *ret)
+  {
+    for (int i = 0; i < k; i++) {
+      ids[(size_t)q * k + i] = (int32_t)(uint32_t)s.r.entry[i];
+      dist2[(size_t)q * k + i] = __uint_as_float((uint32_t)(s.r.entry[i] >> 32));
+    }
+    if (ret) ret[q] = s.r.maxRadius2();
+  }
+};
+
+// HeapCandidateList with a run-time k: the reference's flexible heap list
+// over per-lane storage
+template <> struct ResultOps<cukd::FlexHeapCandidateList> {
+  struct State {
+    uint64_t mem[64];
+    cukd::FlexHeapCandidateList r;
+    __device__ State() : r(mem, 1, -1.f) {}
+  };
+  __device__ static inline cukd::FlexHeapCandidateList &init(State &s, float cutoff, int k)
+  {
+    new (&s.r) cukd::FlexHeapCandidateList(s.mem, k, cutoff);
+    return s.r;
+  }
+  __device__ static inline void write(const State &s, int q, int k, int32_t *ids, float *dist2, float *ret)
+  {
+    for (int i = 0; i < k; i++) {
+      ids[(size_t)q * k + i] = (int32_t)(uint32_t)s.mem[i];
+      dist2[(size_t)q * k + i] = __uint_as_float((uint32_t)(s.mem[i] >> 32));
+    }
+    if (ret) ret[q] = s.r.maxRadius2();
+  }
+};
+
+// ---------------------------------------------------------------------------
+// batch kernel: one query per lane
+// ---------------------------------------------------------------------------
+template <int D, int METHOD, class R>
+__global__ __launch_bounds__(128) void k_query(TreeView t, const float *queries, int nq,
+                                               const float *world_bounds, float cutoff,
+                                               float eps, int k, const int32_t *order,
+                                               int32_t *ids, float *dist2, float *ret,
+                                               unsigned long long *stats)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const int qi = order ? order[i] : i;
+  float q[D];
+#pragma unroll
+  for (int d = 0; d < D; d++) q[d] = queries[(size_t)qi * D + d];
+  typename ResultOps<R>::State st;
+  R &res = ResultOps<R>::init(st, cutoff, k);
+  unsigned visits = 0;
+  if (METHOD == CUKD_STACK_BASED) {
+    cukd::detail::traverse_stack_based<D>(res, q, t, visits);
+  } else if (METHOD == CUKD_STACK_FREE) {
+    cukd::detail::traverse_stack_free<D>(res, q, t, eps, visits);
+  } else {
+    float lo[D], hi[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) { lo[d] = world_bounds[d]; hi[d] = world_bounds[D + d]; }
+    cukd::detail::traverse_cct<D>(res, q, lo, hi, t, visits);
+  }
+  ResultOps<R>::write(st, qi, k, ids, dist2, ret);
+  // CUKD_STATS (common.h:348-360): one count per visited node
+  if (stats) atomicAdd(stats, (unsigned long long)visits);
+}
+
+// ---------------------------------------------------------------------------
+// per-dimension launchers
+// ---------------------------------------------------------------------------
+struct QueryArgs {
+  TreeView t;
+  const float *queries;
+  int nq;
+  const float *world_bounds;
+  float cutoff;
+  float eps;
+  int k;
+  int list_kind;   // -1 = FCP
+  const int32_t *order;
+  int32_t *ids;
+  float *dist2;
+  float *ret;
+  unsigned long long *stats;
+};
+
+template <int D, int METHOD, class R>
+inline hipError_t launch_q(const QueryArgs &a, hipStream_t s)
+{
+  const int nt = 128;
+  const int blocks = (a.nq + nt - 1) / nt;
+  hipLaunchKernelGGL((k_query<D, METHOD, R>), dim3(blocks), dim3(nt), 0, s, a.t, a.queries,
+                     a.nq, a.world_bounds, a.cutoff, a.eps, a.k, a.order, a.ids, a.dist2,
+                     a.ret, a.stats);
+  return hipGetLastError();
+}
+
+template <int D, int METHOD>
+inline hipError_t launch_method(const QueryArgs &a, hipStream_t s)
+{
+  if (a.list_kind < 0) return launch_q<D, METHOD, cukd::FCPResult>(a, s);
+  if (a.list_kind == CUKD_LIST_HEAP) return launch_q<D, METHOD, cukd::FlexHeapCandidateList>(a, s);
+  switch (a.k) {
+    case 1: return launch_q<D, METHOD, cukd::FixedCandidateList<1>>(a, s);
+    case 4: return launch_q<D, METHOD, cukd::FixedCandidateList<4>>(a, s);
+    case 8: return launch_q<D, METHOD, cukd::FixedCandidateList<8>>(a, s);
+    case 16: return launch_q<D, METHOD, cukd::FixedCandidateList<16>>(a, s);
+    default: return launch_q<D, METHOD, DynFixedList>(a, s);
+  }
+}
+
+template <int D>
+inline hipError_t launch_query_d(int method, const QueryArgs &a, hipStream_t s)
+{
+  if (method == CUKD_STACK_BASED) return launch_method<D, CUKD_STACK_BASED>(a, s);
+  if (method == CUKD_STACK_FREE) return launch_method<D, CUKD_STACK_FREE>(a, s);
+  return launch_method<D, CUKD_CCT>(a, s);
+}
+
+hipError_t launch_query(int D, int method, const QueryArgs &a, hipStream_t s);
+
+}  // namespace cukd_amd

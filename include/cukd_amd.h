@@ -1,0 +1,190 @@
+/* cukd_amd.h -- C ABI of the gfx950 (MI355X) k-d tree library.
+ *
+ * This is the drop-in boundary for cudaKDTree's build-and-query path.  Each
+ * entry point names the reference interface it replaces (file:line in the
+ * ingowald/cudaKDTree v1.0.1 tree).  The C++ header API under include/cukd/
+ * maps the reference's templates onto these calls; other languages bind
+ * them directly (see INTEGRATION.md for a ctypes stub).
+ *
+ * Conventions
+ *  - All pointers named d_* are device pointers.  Every call is
+ *    stream-ordered on 'stream' and returns before the GPU work is done,
+ *    except where noted; no call synchronises the device or the stream.
+ *  - Return value: CUKD_OK (0) or a negative CUKD_ERR_* code; the message
+ *    of the last error on the calling thread is in cukd_last_error().
+ *    Nothing throws across this boundary.
+ *  - Sizes are int32 like the reference's 'int numPoints' (N < 2^30,
+ *    the depth limit of the reference's 30-entry traversal stacks).
+ */
+#ifndef CUKD_AMD_H
+#define CUKD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUKD_OK 0
+#define CUKD_ERR_INVALID (-1)     /* bad argument (null pointer, bad layout) */
+#define CUKD_ERR_HIP (-2)         /* a HIP runtime call failed               */
+#define CUKD_ERR_NO_BOUNDS (-3)   /* explicit dims but no world-bounds buffer */
+#define CUKD_ERR_ALLOC (-4)       /* the memory resource failed              */
+#define CUKD_ERR_UNSUPPORTED (-5) /* combination not implemented            */
+
+/* Runtime description of one data_t element, replacing the compile-time
+ * data_traits/point_traits pair (data.h:56-116, cukd-math.h:298-331):
+ * num_dims scalar coordinates stored contiguously at point_offset, an
+ * optional split-dim field (data_traits::has_explicit_dim, get_dim/set_dim,
+ * data.h:98-113) and the element stride.  Covers float2/3/4, int2/3/4,
+ * vec_float<N>, and payload structs such as PointPlusPayload or Photon. */
+typedef struct cukd_layout {
+  int32_t num_dims;     /* 1..8                                         */
+  int32_t scalar_type;  /* CUKD_SCALAR_F32 or CUKD_SCALAR_I32           */
+  int32_t stride;       /* sizeof(data_t), a multiple of 4, <= 256      */
+  int32_t point_offset; /* byte offset of coordinate 0, multiple of 4   */
+  int32_t dim_offset;   /* byte offset of the split-dim field, -1: none */
+  int32_t dim_bytes;    /* 1, 2 or 4                                    */
+} cukd_layout;
+
+#define CUKD_SCALAR_F32 0
+#define CUKD_SCALAR_I32 1
+
+/* Allocation policy for builder temporaries, replacing
+ * cukd::GpuMemoryResource (helpers.h:31-34).  NULL selects the default,
+ * stream-ordered hipMallocAsync/hipFreeAsync (AsyncGpuMemoryResource,
+ * helpers.h:52-60).  Callbacks return 0 on success. */
+typedef struct cukd_mem_resource {
+  void *ctx;
+  int (*malloc_fn)(void *ctx, void **ptr, size_t bytes, hipStream_t s);
+  int (*free_fn)(void *ctx, void *ptr, hipStream_t s);
+} cukd_mem_resource;
+
+/* builder selection (builder.h:108-151) */
+#define CUKD_BUILDER_DEFAULT 0 /* cukd::buildTree                        */
+#define CUKD_BUILDER_THRUST 1  /* cukd::buildTree_thrust                 */
+#define CUKD_BUILDER_BITONIC 2 /* cukd::buildTree_bitonic                */
+#define CUKD_BUILDER_INPLACE 3 /* cukd::buildTree_inPlace                */
+
+/* traversal selection (fcp.h / knn.h namespaces) */
+#define CUKD_STACK_BASED 0 /* cukd::stackBased:: (traverse-default-stack-based.h) */
+#define CUKD_STACK_FREE 1  /* cukd::stackFree::  (traverse-stack-free.h)          */
+#define CUKD_CCT 2         /* cukd::cct::        (traverse-cct.h)                 */
+
+/* candidate lists (knn.h:214-258) */
+#define CUKD_LIST_FIXED 0 /* FixedCandidateList<k>: sorted ascending */
+#define CUKD_LIST_HEAP 1  /* HeapCandidateList<k>: max-heap order    */
+
+/* query flags */
+#define CUKD_QUERY_SORT 1 /* Morton-order the queries before traversal
+                             (results are returned in input order)      */
+
+/* Builds a left-balanced k-d tree in place over d_points[0..n).
+ * Replaces template<data_t,traits> buildTree / buildTree_thrust /
+ * buildTree_bitonic / buildTree_inPlace (builder.h:108-122,
+ * builder_thrust.h:90-110, builder_bitonic.h:83-94,
+ * builder_inplace.h:82-102).  d_world_bounds (box of 2*num_dims scalars,
+ * lower then upper) receives the bounds when non-null and is required for
+ * explicit-dim layouts (builder_thrust.h:469-473).  n < 1 returns CUKD_OK
+ * without work (builder_thrust.h:441).  The resulting array is identical
+ * to buildTree_host's (builder_thrust.h:514-601) for every input without
+ * NaN coordinates, including inputs with tied coordinates. */
+int cukd_build_tree(void *d_points, int32_t n, const cukd_layout *layout,
+                    int builder, void *d_world_bounds, hipStream_t stream,
+                    const cukd_mem_resource *mem);
+
+/* Bytes of temporary device memory cukd_build_tree will request. */
+int cukd_build_tree_temp_bytes(int32_t n, const cukd_layout *layout,
+                               int builder, size_t *bytes);
+
+/* World bounds of d_points (builder_common.h:59-64, 151-163). */
+int cukd_compute_bounds(void *d_bounds, const void *d_points, int32_t n,
+                        const cukd_layout *layout, hipStream_t stream);
+
+/* Batched find-closest-point over a built tree: one query per d_queries
+ * row (num_dims floats).  Replaces per-thread calls of
+ * stackBased/stackFree/cct::fcp (fcp.h:61-163).  d_ids[i] is the tree
+ * index of the closest point with dist2 < cutoff^2, or -1; d_dist2
+ * (nullable) receives that dist2 (cutoff^2 when none).  eps is
+ * FcpSearchParams::eps (stack-free only, fcp.h:237).  d_world_bounds is
+ * required for CUKD_CCT. */
+int cukd_fcp(int traversal, const void *d_tree, int32_t n,
+             const cukd_layout *layout, const void *d_world_bounds,
+             const float *d_queries, int32_t nq, float cutoff, float eps,
+             int32_t *d_ids, float *d_dist2, int flags, hipStream_t stream,
+             const cukd_mem_resource *mem);
+
+/* Batched k-nearest-neighbour query.  Replaces per-thread calls of
+ * stackBased/stackFree/cct::knn with a FixedCandidateList<k> or
+ * HeapCandidateList<k> (knn.h:83-199, 214-258).  For query i the k list
+ * entries are written to d_ids[i*k..] (int(uint32(entry)), -1 = empty) and
+ * d_dist2[i*k..], in the list's own storage order; d_max_radius2[i]
+ * (nullable) receives the call's return value, maxRadius2().  1 <= k <= 64. */
+int cukd_knn(int traversal, int list_kind, int k, const void *d_tree,
+             int32_t n, const cukd_layout *layout,
+             const void *d_world_bounds, const float *d_queries, int32_t nq,
+             float cutoff, int32_t *d_ids, float *d_dist2,
+             float *d_max_radius2, int flags, hipStream_t stream,
+             const cukd_mem_resource *mem);
+
+/* Spatial k-d tree (spatial-kdtree.h:69-98).  Node = {float pos;
+ * uint32 offset; uint16 count; int16 dim} (12 bytes). */
+typedef struct cukd_spatial_node {
+  float pos;
+  uint32_t offset;
+  uint16_t count;
+  int16_t dim;
+} cukd_spatial_node;
+
+typedef struct cukd_spatial_tree {
+  float bounds[2 * 8];           /* lower[num_dims] then upper[num_dims] */
+  cukd_spatial_node *nodes;      /* device, num_nodes entries            */
+  uint32_t *prim_ids;            /* device, num_prims entries            */
+  const void *data;              /* device, the (unpermuted) points      */
+  int32_t num_prims;
+  int32_t num_nodes;
+} cukd_spatial_tree;
+
+/* Builds a spatial k-d tree (spatial-kdtree.h:113-120, 496-612, 632-645).
+ * leaf_threshold 0 means 8 (BuildConfig::makeLeafThreshold, :100-105).
+ * Synchronises 'stream' before returning, as the reference does, because
+ * the node count and bounds are returned in host memory. */
+int cukd_spatial_build(cukd_spatial_tree *tree, const void *d_points,
+                       int32_t n, const cukd_layout *layout,
+                       int leaf_threshold, hipStream_t stream,
+                       const cukd_mem_resource *mem);
+
+/* Releases a spatial tree's nodes/primIDs (spatial-kdtree.h:618-630). */
+int cukd_spatial_free(cukd_spatial_tree *tree, hipStream_t stream,
+                      const cukd_mem_resource *mem);
+
+/* Spatial FCP / kNN (fcp.h:256-427, knn.h:566-752); traversal is
+ * CUKD_STACK_BASED or CUKD_CCT.  Ids are original point indices. */
+int cukd_spatial_fcp(int traversal, const cukd_spatial_tree *tree,
+                     const cukd_layout *layout, const float *d_queries,
+                     int32_t nq, float cutoff, int32_t *d_ids, float *d_dist2,
+                     hipStream_t stream);
+int cukd_spatial_knn(int traversal, int list_kind, int k,
+                     const cukd_spatial_tree *tree, const cukd_layout *layout,
+                     const float *d_queries, int32_t nq, float cutoff,
+                     int32_t *d_ids, float *d_dist2, float *d_max_radius2,
+                     hipStream_t stream);
+
+/* Traversal statistics, replacing CUKD_STATS / cukd::g_traversalStats
+ * (common.h:348-360): while d_counter is non-null, every query launch adds
+ * the number of tree nodes its traversals visited (one per visited node,
+ * as at traverse-default-stack-based.h:64) to *d_counter.  NULL disables. */
+int cukd_set_traversal_stats(unsigned long long *d_counter);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *cukd_last_error(void);
+
+/* Library version, 100*major + minor. */
+int cukd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

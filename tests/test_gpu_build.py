@@ -1,0 +1,173 @@
+"""GPU builder parity: the tree cukd_build_tree writes is byte-identical to
+the oracle's buildTree_host restatement (builder_thrust.h:514-601), for every
+builder entry point, element layout and tie pattern."""
+import numpy as np
+import pytest
+import torch
+
+import cudakdtree_amd as ck
+import oracle as oc
+from cudakdtree_amd import datagen
+from tests.helpers import (check_kdtree_valid, coords_of, dims_of, distribution,
+                           layout_for, make_elements)
+
+pytestmark = pytest.mark.gpu
+
+BUILDERS = [ck.BUILDER_DEFAULT, ck.BUILDER_THRUST, ck.BUILDER_BITONIC, ck.BUILDER_INPLACE]
+
+
+def gpu_build(elems, layout, builder=ck.BUILDER_DEFAULT, want_bounds=False):
+    t = torch.from_numpy(np.ascontiguousarray(elems)).cuda()
+    b = ck.bounds_tensor(layout) if (want_bounds or layout.explicit_dim) else None
+    ck.build_tree(t, layout, builder, b)
+    torch.cuda.synchronize()
+    return t.cpu().numpy(), (None if b is None else b.cpu().numpy())
+
+
+def compare(coords, kind, builder=ck.BUILDER_DEFAULT, want_bounds=True):
+    lay = layout_for(kind, coords.shape[1])
+    elems = make_elements(coords, lay)
+    ref, rbox = oc.build_host(elems, lay, want_bounds=True)
+    got, gbox = gpu_build(elems, lay, builder, want_bounds)
+    if not np.array_equal(got, ref):
+        bad = np.nonzero(np.any(got != ref, axis=1))[0]
+        raise AssertionError("%d of %d nodes differ, first at %d" % (len(bad), len(ref), bad[0]))
+    if gbox is not None:
+        # compare as scalars: -0.0 == +0.0 (box_t::grow keeps either)
+        assert np.array_equal(gbox.view(rbox.dtype), rbox)
+    return lay, got
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 7, 8, 9, 100, 511, 512, 513, 1000, 4095, 5000,
+                               8191, 8192, 8193, 20000, 65537])
+def test_float3_uniform_sizes(n):
+    compare(datagen.uniform(n, 3, n), "float")
+
+
+@pytest.mark.parametrize("builder", BUILDERS)
+@pytest.mark.parametrize("dist", ["uniform", "ties", "dups", "plane", "gmm", "signed_zero",
+                                  "unique"])
+def test_float3_distributions(builder, dist):
+    compare(distribution(dist, 30011, 3, seed=2), "float", builder)
+
+
+@pytest.mark.parametrize("D", [1, 2, 3, 4, 5, 8])
+@pytest.mark.parametrize("kind", ["float", "int", "payload", "payload_int"])
+def test_layouts_round_robin(D, kind):
+    scalar_float = kind in ("float", "payload")
+    coords = distribution("ties" if D > 2 else "uniform", 12345, D, seed=D,
+                          scalar_float=scalar_float)
+    compare(coords, kind)
+
+
+@pytest.mark.parametrize("D", [2, 3, 4, 8])
+@pytest.mark.parametrize("kind", ["explicit", "explicit_int", "photon"])
+@pytest.mark.parametrize("dist", ["uniform", "ties", "gmm"])
+@pytest.mark.parametrize("n", [1, 6, 3000, 40000])
+def test_explicit_dims(D, kind, dist, n):
+    scalar_float = kind != "explicit_int"
+    coords = distribution(dist, n, D, seed=n + D, scalar_float=scalar_float)
+    lay, got = compare(coords, kind)
+    check_kdtree_valid(coords_of(got, lay), dims_of(got, lay), n)
+
+
+@pytest.mark.parametrize("D", [2, 3, 4, 8])
+@pytest.mark.parametrize("kind", ["float", "int", "payload", "explicit", "explicit_int"])
+@pytest.mark.parametrize("n", [4, 10, 1000, 10000])
+def test_builders_same_result_matrix(D, kind, n):
+    """testing/testBuildersSameResult.cu:404-430: tie-free unique-int
+    coordinates, all builders equal the host builder."""
+    c = datagen.unique_ints(n, D, seed=n + 7 * D)
+    coords = c.astype(np.float32) if kind in ("float", "payload", "explicit") else c
+    for b in BUILDERS:
+        compare(coords, kind, b)
+
+
+def test_two_million_uniform_and_gmm():
+    for dist in ("uniform", "gmm"):
+        compare(distribution(dist, 2_000_000, 3, seed=77), "float")
+
+
+def test_repeat_builds_identical():
+    coords = distribution("ties", 200_000, 3, seed=4)
+    lay = ck.float_points(3)
+    a, _ = gpu_build(coords, lay)
+    b, _ = gpu_build(coords, lay)
+    assert np.array_equal(a, b)
+
+
+def test_user_stream_and_memory_resource():
+    coords = datagen.uniform(100_000, 3, 5)
+    lay = ck.float_points(3)
+    ref, _ = oc.build_host(coords, lay)
+    calls = {"malloc": 0, "free": 0}
+    keep = {}
+
+    @ck.MemResource._fields_[1][1]
+    def _malloc(ctx, pptr, nbytes, stream):
+        t = torch.empty(int(nbytes), dtype=torch.uint8, device="cuda")
+        keep[t.data_ptr()] = t
+        pptr[0] = t.data_ptr()
+        calls["malloc"] += 1
+        return 0
+
+    @ck.MemResource._fields_[2][1]
+    def _free(ctx, ptr, stream):
+        calls["free"] += 1
+        return 0
+
+    mr = ck.MemResource(None, _malloc, _free)
+    s = torch.cuda.Stream()
+    t = torch.from_numpy(coords.copy()).cuda()
+    torch.cuda.synchronize()
+    import ctypes
+    rc = ck.lib().cukd_build_tree(ctypes.c_void_p(t.data_ptr()), len(coords), ctypes.byref(lay),
+                                  0, None, ctypes.c_void_p(s.cuda_stream), ctypes.byref(mr))
+    assert rc == 0
+    s.synchronize()
+    assert calls["malloc"] == 1 and calls["free"] == 1
+    assert np.array_equal(t.cpu().numpy(), ref)
+
+
+def test_memory_resource_failure_reports_error():
+    import ctypes
+    lay = ck.float_points(3)
+
+    @ck.MemResource._fields_[1][1]
+    def _malloc(ctx, pptr, nbytes, stream):
+        return 1
+
+    @ck.MemResource._fields_[2][1]
+    def _free(ctx, ptr, stream):
+        return 0
+
+    mr = ck.MemResource(None, _malloc, _free)
+    t = torch.zeros((1000, 3), dtype=torch.float32, device="cuda")
+    rc = ck.lib().cukd_build_tree(ctypes.c_void_p(t.data_ptr()), 1000, ctypes.byref(lay), 0,
+                                  None, None, ctypes.byref(mr))
+    assert rc == ck.ERR_ALLOC
+    assert b"allocate" in ck.lib().cukd_last_error()
+
+
+def test_compute_bounds():
+    coords = distribution("gmm", 123457, 4, seed=3)
+    lay = ck.float_points(4)
+    b = ck.bounds_tensor(lay)
+    ck.computeBounds(b, torch.from_numpy(coords).cuda(), lay)
+    np.testing.assert_array_equal(b.cpu().numpy(), oc.compute_bounds(coords, lay))
+
+
+@pytest.mark.slow
+def test_ten_million_properties():
+    """Full bench size: a valid k-d tree and a permutation of the input
+    (the oracle's 10M build is minutes on one core; the 2M case above
+    pins exact parity on the same code path)."""
+    n = 10_000_000
+    coords = datagen.uniform(n, 3, 1)
+    lay = ck.float_points(3)
+    got, _ = gpu_build(coords, lay)
+    lv = np.floor(np.log2(np.arange(n, dtype=np.float64) + 1)).astype(np.int64) % 3
+    check_kdtree_valid(got, lv, n)
+    a = coords.view(np.uint32)
+    b = got.view(np.uint32)
+    assert np.array_equal(a[np.lexsort(a.T)], b[np.lexsort(b.T)])
