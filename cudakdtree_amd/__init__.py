@@ -291,3 +291,94 @@ def _ns(traversal):
 stackBased = _ns(STACK_BASED)
 stackFree = _ns(STACK_FREE)
 cct = _ns(CCT)
+
+
+# ----------------------------------------------------------------------------
+# spatial k-d tree (spatial-kdtree.h:69-126)
+# ----------------------------------------------------------------------------
+
+class SpatialKDTree:
+    """Device-resident spatial k-d tree; ``bounds`` is a host list of
+    2*num_dims floats.  Keeps the point tensor alive (the tree references
+    it, the reference's SpatialKDTree::data)."""
+
+    def __init__(self, raw, points, layout):
+        self.raw = raw
+        self.points = points
+        self.layout = layout
+
+    @property
+    def num_nodes(self):
+        return self.raw.num_nodes
+
+    @property
+    def num_prims(self):
+        return self.raw.num_prims
+
+    @property
+    def bounds(self):
+        return list(self.raw.bounds[:2 * self.layout.num_dims])
+
+    def nodes_numpy(self):
+        """Copies the node array to the host as a structured array."""
+        import torch
+        n = self.raw.num_nodes
+        buf = torch.empty(n * 12, dtype=torch.uint8, device=self.points.device)
+        _copy_dev(buf, self.raw.nodes, n * 12)
+        dt = np.dtype([("pos", "<f4"), ("offset", "<u4"), ("count", "<u2"), ("dim", "<i2")])
+        return buf.cpu().numpy().view(dt)
+
+    def prims_numpy(self):
+        import torch
+        n = self.raw.num_prims
+        buf = torch.empty(max(n, 1), dtype=torch.int32, device=self.points.device)
+        if n:
+            _copy_dev(buf, self.raw.prim_ids, n * 4)
+        return buf.cpu().numpy().view(np.uint32)[:n]
+
+
+def _copy_dev(dst_tensor, src_ptr, nbytes):
+    import torch
+    torch.cuda.synchronize()
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    rc = hip.hipMemcpy(ctypes.c_void_p(dst_tensor.data_ptr()), ctypes.c_void_p(src_ptr), nbytes, 3)
+    if rc != 0:
+        raise RuntimeError("hipMemcpy failed (%d)" % rc)
+
+
+def buildSpatialTree(points, layout, leaf_threshold=0, stream=None):
+    """cukd::buildTree(SpatialKDTree&, ...): synchronises the stream."""
+    n = _rows(points, layout)
+    raw = SpatialTree()
+    _check(lib().cukd_spatial_build(ctypes.byref(raw), _ptr(points), n, ctypes.byref(layout),
+                                    leaf_threshold, _stream(stream), None))
+    return SpatialKDTree(raw, points, layout)
+
+
+def free(tree, stream=None):
+    """cukd::free(SpatialKDTree&)."""
+    _check(lib().cukd_spatial_free(ctypes.byref(tree.raw), _stream(stream), None))
+
+
+def spatial_fcp(tree, queries, traversal=STACK_BASED, cutoff=math.inf, stream=None):
+    import torch
+    nq = queries.shape[0]
+    ids = torch.empty(nq, dtype=torch.int32, device=queries.device)
+    d2 = torch.empty(nq, dtype=torch.float32, device=queries.device)
+    _check(lib().cukd_spatial_fcp(traversal, ctypes.byref(tree.raw), ctypes.byref(tree.layout),
+                                  _ptr(queries), nq, cutoff, _ptr(ids), _ptr(d2), _stream(stream)))
+    return ids, d2
+
+
+def spatial_knn(tree, queries, k, traversal=STACK_BASED, list_kind=LIST_FIXED,
+                cutoff=math.inf, stream=None):
+    import torch
+    nq = queries.shape[0]
+    ids = torch.empty((nq, k), dtype=torch.int32, device=queries.device)
+    d2 = torch.empty((nq, k), dtype=torch.float32, device=queries.device)
+    ret = torch.empty(nq, dtype=torch.float32, device=queries.device)
+    _check(lib().cukd_spatial_knn(traversal, list_kind, k, ctypes.byref(tree.raw),
+                                  ctypes.byref(tree.layout), _ptr(queries), nq, cutoff, _ptr(ids),
+                                  _ptr(d2), _ptr(ret), _stream(stream)))
+    return ids, d2, ret

@@ -299,31 +299,101 @@ int cukd_knn(int traversal, int list_kind, int k, const void *d_tree, int32_t n,
   return CUKD_OK;
 }
 
-int cukd_spatial_build(cukd_spatial_tree *, const void *, int32_t, const cukd_layout *, int,
-                       hipStream_t, const cukd_mem_resource *)
+int cukd_spatial_build(cukd_spatial_tree *tree, const void *d_points, int32_t n,
+                       const cukd_layout *layout, int leaf_threshold, hipStream_t stream,
+                       const cukd_mem_resource *mem)
 {
-  set_last_error("spatial k-d tree not built into this library version");
-  return CUKD_ERR_UNSUPPORTED;
+  g_last_error[0] = 0;
+  if (int rc = check_layout(layout)) return rc;
+  if (!tree) { set_last_error("tree is null"); return CUKD_ERR_INVALID; }
+  if (layout->scalar_type != CUKD_SCALAR_F32) {
+    set_last_error("spatial k-d trees need float coordinates");
+    return CUKD_ERR_UNSUPPORTED;
+  }
+  if (n < 0 || (n > 0 && !d_points)) { set_last_error("bad points"); return CUKD_ERR_INVALID; }
+  if (leaf_threshold > 65535) {
+    set_last_error("leaf threshold %d exceeds the 16-bit leaf count", leaf_threshold);
+    return CUKD_ERR_INVALID;
+  }
+  MemResource mr{mem};
+  return spatial_build_impl(tree, d_points, n, *layout, leaf_threshold, stream, mr);
 }
 
-int cukd_spatial_free(cukd_spatial_tree *, hipStream_t, const cukd_mem_resource *)
+int cukd_spatial_free(cukd_spatial_tree *tree, hipStream_t stream, const cukd_mem_resource *mem)
 {
-  set_last_error("spatial k-d tree not built into this library version");
-  return CUKD_ERR_UNSUPPORTED;
+  g_last_error[0] = 0;
+  if (!tree) { set_last_error("tree is null"); return CUKD_ERR_INVALID; }
+  MemResource mr{mem};
+  mr.release(tree->nodes, stream);
+  mr.release(tree->prim_ids, stream);
+  tree->nodes = nullptr;
+  tree->prim_ids = nullptr;
+  tree->num_nodes = 0;
+  tree->num_prims = 0;
+  return CUKD_OK;
 }
 
-int cukd_spatial_fcp(int, const cukd_spatial_tree *, const cukd_layout *, const float *,
-                     int32_t, float, int32_t *, float *, hipStream_t)
+static int spatial_query(int traversal, int list_kind, int k, const cukd_spatial_tree *tree,
+                         const cukd_layout *layout, const float *d_queries, int32_t nq,
+                         float cutoff, int32_t *d_ids, float *d_dist2, float *d_ret,
+                         hipStream_t stream)
 {
-  set_last_error("spatial k-d tree not built into this library version");
-  return CUKD_ERR_UNSUPPORTED;
+  if (int rc = check_layout(layout)) return rc;
+  if (layout->scalar_type != CUKD_SCALAR_F32) {
+    set_last_error("spatial k-d trees need float coordinates");
+    return CUKD_ERR_UNSUPPORTED;
+  }
+  if (traversal != CUKD_STACK_BASED && traversal != CUKD_CCT) {
+    set_last_error("spatial trees support stack-based and cct traversal");
+    return CUKD_ERR_INVALID;
+  }
+  if (!tree || nq < 0) { set_last_error("bad arguments"); return CUKD_ERR_INVALID; }
+  if (nq == 0) return CUKD_OK;
+  if (!tree->nodes || !d_queries || !d_ids) { set_last_error("null pointer"); return CUKD_ERR_INVALID; }
+  SpatialArgs a;
+  a.nodes = tree->nodes;
+  a.prim_ids = tree->prim_ids;
+  a.data = make_view(tree->data, tree->num_prims, layout);
+  for (int i = 0; i < 2 * CUKD_MAX_DIMS; i++) a.bounds.v[i] = tree->bounds[i];
+  a.queries = d_queries;
+  a.nq = nq;
+  a.cutoff = cutoff;
+  a.k = k;
+  a.list_kind = list_kind;
+  a.ids = d_ids;
+  a.dist2 = d_dist2;
+  a.ret = d_ret;
+  const hipError_t e = launch_spatial_query(layout->num_dims, traversal, a, stream);
+  if (e != hipSuccess) {
+    set_last_error("spatial query launch failed: %s", hipGetErrorString(e));
+    return CUKD_ERR_HIP;
+  }
+  return CUKD_OK;
 }
 
-int cukd_spatial_knn(int, int, int, const cukd_spatial_tree *, const cukd_layout *,
-                     const float *, int32_t, float, int32_t *, float *, float *, hipStream_t)
+int cukd_spatial_fcp(int traversal, const cukd_spatial_tree *tree, const cukd_layout *layout,
+                     const float *d_queries, int32_t nq, float cutoff, int32_t *d_ids,
+                     float *d_dist2, hipStream_t stream)
 {
-  set_last_error("spatial k-d tree not built into this library version");
-  return CUKD_ERR_UNSUPPORTED;
+  g_last_error[0] = 0;
+  return spatial_query(traversal, -1, 1, tree, layout, d_queries, nq, cutoff, d_ids, d_dist2,
+                       nullptr, stream);
+}
+
+int cukd_spatial_knn(int traversal, int list_kind, int k, const cukd_spatial_tree *tree,
+                     const cukd_layout *layout, const float *d_queries, int32_t nq,
+                     float cutoff, int32_t *d_ids, float *d_dist2, float *d_max_radius2,
+                     hipStream_t stream)
+{
+  g_last_error[0] = 0;
+  if (k < 1 || k > 64) { set_last_error("k = %d not in 1..64", k); return CUKD_ERR_INVALID; }
+  if (list_kind != CUKD_LIST_FIXED && list_kind != CUKD_LIST_HEAP) {
+    set_last_error("unknown list kind %d", list_kind);
+    return CUKD_ERR_INVALID;
+  }
+  if (nq > 0 && !d_dist2) { set_last_error("d_dist2 is null"); return CUKD_ERR_INVALID; }
+  return spatial_query(traversal, list_kind, k, tree, layout, d_queries, nq, cutoff, d_ids,
+                       d_dist2, d_max_radius2, stream);
 }
 
 }  // extern "C"

@@ -232,4 +232,26 @@ inline hipError_t launch_query_d(int method, const QueryArgs &a, hipStream_t s)
 
 hipError_t launch_query(int D, int method, const QueryArgs &a, hipStream_t s);
 
+// spatial-tree queries (spatial.hip)
+struct SpatialBounds {
+  float v[2 * CUKD_MAX_DIMS];
+};
+struct SpatialArgs {
+  const cukd_spatial_node *nodes;
+  const uint32_t *prim_ids;
+  TreeView data;
+  SpatialBounds bounds;
+  const float *queries;
+  int nq;
+  float cutoff;
+  int k;
+  int list_kind;  // -1 = FCP
+  int32_t *ids;
+  float *dist2;
+  float *ret;
+};
+hipError_t launch_spatial_query(int D, int method, const SpatialArgs &a, hipStream_t s);
+int spatial_build_impl(cukd_spatial_tree *tree, const void *d_points, int n, const cukd_layout &L,
+                       int leaf_threshold, hipStream_t s, const MemResource &mem);
+
 }  // namespace cukd_amd
