@@ -241,6 +241,13 @@ __device__ inline int tiles_of(const LevelGeom &g, int s, int T)
 // One digit pass of the radix select.  Every tile histograms the digit of
 // its candidates; the last tile of each subtree picks the digit bin that
 // holds the pivot rank.
+//
+// Cross-workgroup protocol without device fences (an agent-scope fence
+// writes back and invalidates L2, which thrashes the cache for every other
+// tile): a tile adds its bins with value-returning atomics, so they have
+// been performed before the tile's barrier and before its thread 0 bumps
+// the subtree's tile counter; the tile that sees the final count reads and
+// clears the bins with atomic exchanges.
 __global__ __launch_bounds__(SEL_NT) void k_select_pass(
     LevelGeom g, int word, int shift, int bits, int T,
     const uint32_t *key, const uint8_t *data, const uint32_t *idx,
@@ -253,6 +260,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
   __shared__ SegState s_st[2];
   __shared__ int s_seg[2];
   __shared__ int s_last[2];
+  __shared__ uint32_t s_sink;
   const bool isf = L.scalar_type == CUKD_SCALAR_F32;
   const int nbins = 1 << bits;
   const int t0 = blockIdx.x * T;
@@ -260,7 +268,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
   const int seg0 = g.seg_of(t0);
   const int seg1 = g.seg_of(t1 - 1);
   for (int i = threadIdx.x; i < 2 * nbins; i += blockDim.x) s_hist[i / nbins][i % nbins] = 0u;
-  if (threadIdx.x == 0) { s_seg[0] = seg0; s_seg[1] = seg1; s_st[0] = st[seg0]; }
+  if (threadIdx.x == 0) { s_seg[0] = seg0; s_seg[1] = seg1; s_st[0] = st[seg0]; s_sink = 0; }
   if (threadIdx.x == 1) s_st[1] = st[seg1];
   __syncthreads();
   // tiles inside one subtree use slot 0 only
@@ -271,27 +279,37 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
     const int slot = p >= seg1_begin ? 1 : 0;
     const SegState &t = s_st[slot];
     if (t.resolved || (int)t.word != word) continue;
-    const uint8_t *elem = data + (size_t)p * L.stride;
-    const uint32_t id = idx[p];
     const uint32_t k0 = key[p];
-    bool cand = true;
-    for (int w = 0; w < word && cand; w++)
-      cand = key_word(w, t, k0, elem, id, L, isf) == t.fixed[w];
-    if (!cand) continue;
-    const uint32_t v = key_word(word, t, k0, elem, id, L, isf);
+    uint32_t v;
+    if (word == 0) {
+      v = k0;
+    } else {
+      // candidates agree with the pivot on every earlier word
+      if (k0 != t.fixed[0]) continue;
+      const uint8_t *elem = data + (size_t)p * L.stride;
+      const uint32_t id = idx[p];
+      bool cand = true;
+      for (int w = 1; w < word && cand; w++)
+        cand = key_word(w, t, k0, elem, id, L, isf) == t.fixed[w];
+      if (!cand) continue;
+      v = key_word(word, t, k0, elem, id, L, isf);
+    }
     if (hi_shift < 32 && (v >> hi_shift) != t.prefix) continue;
     atomicAdd(&s_hist[slot][(v >> shift) & mask], 1u);
   }
   __syncthreads();
   const int nslots = seg1 != seg0 ? 2 : 1;
+  uint32_t sink = 0;
   for (int sl = 0; sl < nslots; sl++) {
     const int s = s_seg[sl];
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
       const uint32_t c = s_hist[sl][b];
-      if (c) atomicAdd(&hist[(size_t)s * DIGIT_MAX_BINS + b], c);
+      if (c) sink |= atomicAdd(&hist[(size_t)s * DIGIT_MAX_BINS + b], c);
     }
   }
-  __threadfence();
+  // consuming the returned values makes every atomic complete before the
+  // barrier below
+  if (sink == 0xffffffffu) s_sink = sink;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int sl = 0; sl < 2; sl++) {
@@ -306,14 +324,10 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
   __syncthreads();
   for (int sl = 0; sl < nslots; sl++) {
     if (!s_last[sl]) continue;
-    __threadfence();
     const int s = s_seg[sl];
     uint32_t *h = hist + (size_t)s * DIGIT_MAX_BINS;
-    // reuse s_hist[sl] for the subtree's global histogram
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
-      s_hist[sl][b] = __hip_atomic_load(&h[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      h[b] = 0u;
-    }
+    // reuse s_hist[sl] for the subtree's global histogram (read + clear)
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[sl][b] = atomicExch(&h[b], 0u);
     __syncthreads();
     if (threadIdx.x == 0) {
       SegState t = s_st[sl];
@@ -460,32 +474,89 @@ __global__ __launch_bounds__(SEL_NT) void k_partition(
 // LDS finish: one workgroup builds a whole subtree of at most CAP-1 nodes.
 // ---------------------------------------------------------------------------
 
-// Bitonic sort of keys[0..n) as independent ascending blocks of size B
-// (n and B powers of two).
-template <int NT>
-__device__ inline void bitonic_blocks(uint64_t *keys, int n, int B)
-{
-  for (int k = 2; k <= B; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < (n >> 1); i += NT) {
-        const int lo = 2 * i - (i & (j - 1));
-        const int hi = lo + j;
-        const bool asc = (k == B) || ((lo & k) == 0);
-        const uint64_t a = keys[lo], b = keys[hi];
-        if ((a > b) == asc) { keys[lo] = b; keys[hi] = a; }
+// Register-blocked bitonic sort of CAP 64-bit keys (with a 32-bit payload)
+// held E = CAP/NT per thread, thread t owning positions t*E .. t*E+E-1;
+// sorts independent aligned blocks of size B ascending.  Strides below E
+// stay in registers, strides below 64*E use wave shuffles, larger ones go
+// through LDS.  Keys are unique apart from the ~0 padding.
+template <int CAP, int NT>
+struct BlockSort {
+  enum { E = CAP / NT };
+
+  // keep min (true) or max of (key, other) at position i
+  static __device__ inline void cmpx(uint64_t &k, uint32_t &v, uint64_t ok, uint32_t ov, bool keep_min)
+  {
+    const bool take = keep_min ? (ok < k) : (ok > k);
+    k = take ? ok : k;
+    v = take ? ov : v;
+  }
+
+  template <int J>
+  static __device__ inline void reg_stage(uint64_t (&k)[E], uint32_t (&v)[E], int k_size, int B)
+  {
+    const int base = threadIdx.x * E;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      if (e & J) continue;
+      const int i = base + e;
+      const bool asc = (k_size == B) || ((i & k_size) == 0);
+      const uint64_t a = k[e], b = k[e + J];
+      const bool sw = asc ? (a > b) : (a < b);
+      if (sw) {
+        k[e] = b; k[e + J] = a;
+        const uint32_t t = v[e]; v[e] = v[e + J]; v[e + J] = t;
       }
-      __syncthreads();
     }
   }
-}
+
+  static __device__ inline void sort(uint64_t (&k)[E], uint32_t (&v)[E], int B, uint64_t *sx,
+                                     uint32_t *sv)
+  {
+    const int base = threadIdx.x * E;
+    for (int ks = 2; ks <= B; ks <<= 1) {
+      for (int j = ks >> 1; j > 0; j >>= 1) {
+        if (j < E) {
+          if (j >= 8) { if (E > 8) reg_stage<(E > 8 ? 8 : 1)>(k, v, ks, B); }
+          else if (j == 4) { if (E > 4) reg_stage<(E > 4 ? 4 : 1)>(k, v, ks, B); }
+          else if (j == 2) { if (E > 2) reg_stage<(E > 2 ? 2 : 1)>(k, v, ks, B); }
+          else { reg_stage<1>(k, v, ks, B); }
+          continue;
+        }
+        const int lx = j / E;  // partner thread = tid ^ lx, same e
+        if (lx < 64) {
+#pragma unroll
+          for (int e = 0; e < E; e++) {
+            const uint64_t ok = __shfl_xor(k[e], lx);
+            const uint32_t ov = __shfl_xor(v[e], lx);
+            const int i = base + e;
+            const bool asc = (ks == B) || ((i & ks) == 0);
+            cmpx(k[e], v[e], ok, ov, ((i & j) == 0) == asc);
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < E; e++) { sx[base + e] = k[e]; sv[base + e] = v[e]; }
+          __syncthreads();
+#pragma unroll
+          for (int e = 0; e < E; e++) {
+            const int i = base + e;
+            const bool asc = (ks == B) || ((i & ks) == 0);
+            cmpx(k[e], v[e], sx[i ^ j], sv[i ^ j], ((i & j) == 0) == asc);
+          }
+          __syncthreads();
+        }
+      }
+    }
+  }
+};
 
 template <int CAP, int NT, bool EXPLICIT>
 __global__ __launch_bounds__(NT) void k_finish(
     int n, int L0, const uint8_t *data, const uint32_t *idx,
     const SegState *st, uint8_t *out, cukd_layout L)
 {
-  __shared__ uint64_t s_key[CAP];
-  __shared__ uint16_t s_slot[2][CAP];
+  enum { E = CAP / NT };
+  __shared__ uint64_t s_x[CAP];
+  __shared__ uint32_t s_xv[CAP];  // sort exchange, then the slot scatter
   __shared__ uint32_t s_split[EXPLICIT ? CAP : 1];
   __shared__ uint8_t s_sdim[EXPLICIT ? CAP : 1];
   __shared__ uint8_t s_bdim[EXPLICIT ? CAP / 2 : 1];
@@ -495,7 +566,7 @@ __global__ __launch_bounds__(NT) void k_finish(
   const int D = L.num_dims;
   const int h = num_levels_for(n);
   const int hs = h - L0;            // levels in this subtree
-  const int Cs = 1 << hs;           // aligned capacity
+  const int Cs = 1 << hs;           // aligned capacity (<= CAP)
   const int s = blockIdx.x;
   LevelGeom g;
   g.init(n, L0);
@@ -507,15 +578,18 @@ __global__ __launch_bounds__(NT) void k_finish(
   const SegState &gst = st[s];
   if (EXPLICIT && threadIdx.x < 2 * D) s_box[threadIdx.x] = gst.box[threadIdx.x];
 
-  int cur = 0;
-  for (int p = threadIdx.x; p < Cs; p += NT) s_slot[0][p] = (uint16_t)p;
-  __syncthreads();
+  const int p0 = threadIdx.x * E;
+  uint64_t key[E];
+  uint32_t slot[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) slot[e] = (uint32_t)(p0 + e);
+  __syncthreads();  // s_box
 
-  // Entry order.  The host builder's order inside this subtree before its
+  // Entry order: the host builder's order inside this subtree before its
   // level-L0 sort is the composite key of the levels above: the subtree's
-  // own chain without its first dim, then the input index.  Establish it
-  // with LSD passes: input index first, then the chain dims from least to
-  // most significant, each tie-broken by the current position.
+  // own chain without its first dim, then the input index.  LSD passes:
+  // (last chain coordinate, input index) first, then the chain dims from
+  // least to most significant, each tie-broken by the current position.
   if (L0 > 0) {
     uint32_t chain = 0, len = 0;
     if (EXPLICIT) {
@@ -524,24 +598,26 @@ __global__ __launch_bounds__(NT) void k_finish(
     } else {
       for (int l = L0; l >= 0 && (int)len < D; l--) chain |= (uint32_t)(l % D) << (4 * len++);
     }
-    for (int j = (int)len; j >= 1; j--) {
-      const int d = j < (int)len ? (int)chain_at(chain, j) : -1;
-      for (int p = threadIdx.x; p < Cs; p += NT) {
-        uint64_t k = ~0ull;
+    for (int j = (int)len - 1; j >= 0; j--) {
+      const bool first = j == (int)len - 1;
+      if (j == 0 && !first) break;  // chain[0] is this level's own dim
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        const int p = p0 + e;
+        uint64_t kk = ~0ull;
         if (p < m) {
-          const int sl = s_slot[cur][p];
-          const uint64_t hi = d < 0 ? (uint64_t)sidx[sl]
-              : (uint64_t)ordered_key(load_coord_bits(sdata + (size_t)sl * L.stride, L, d), isf);
-          k = (hi << 32) | (uint64_t)p;
+          const uint32_t sl = slot[e];
+          if (first) {
+            const uint64_t hi = j >= 1 ? (uint64_t)ordered_key(load_coord_bits(sdata + (size_t)sl * L.stride, L, (int)chain_at(chain, j)), isf) : 0ull;
+            kk = (hi << 32) | (uint64_t)sidx[sl];
+          } else {
+            kk = ((uint64_t)ordered_key(load_coord_bits(sdata + (size_t)sl * L.stride, L, (int)chain_at(chain, j)), isf) << 32) | (uint64_t)p;
+          }
         }
-        s_key[p] = k;
+        key[e] = kk;
       }
-      __syncthreads();
-      bitonic_blocks<NT>(s_key, Cs, Cs);
-      for (int q = threadIdx.x; q < m; q += NT)
-        s_slot[cur ^ 1][q] = s_slot[cur][(int)(uint32_t)s_key[q]];
-      __syncthreads();
-      cur ^= 1;
+      BlockSort<CAP, NT>::sort(key, slot, Cs, s_x, s_xv);
+      if (j == 0) break;
     }
   }
 
@@ -549,7 +625,6 @@ __global__ __launch_bounds__(NT) void k_finish(
     const int B = Cs >> j;
     const int nblk = 1 << j;
     const int first_node = (g_node + 1) * nblk - 1;
-    // dims of this depth's blocks
     if (EXPLICIT) {
       for (int b = threadIdx.x; b < nblk; b += NT) {
         if (first_node + b >= n) { s_bdim[b] = 0; continue; }  // no such node
@@ -576,28 +651,30 @@ __global__ __launch_bounds__(NT) void k_finish(
       __syncthreads();
     }
     const int rr_dim = (L0 + j) % D;
-    for (int p = threadIdx.x; p < Cs; p += NT) {
-      const int b = p / B;
-      const int node = first_node + b;
-      const int sz = subtree_size(node, n);
-      uint64_t k = ~0ull;
-      if (p - b * B < sz) {
-        const int d = EXPLICIT ? (int)s_bdim[b] : rr_dim;
-        const int sl = s_slot[cur][p];
-        k = ((uint64_t)ordered_key(load_coord_bits(sdata + (size_t)sl * L.stride, L, d), isf) << 32) | (uint64_t)p;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int p = p0 + e;
+      uint64_t kk = ~0ull;
+      if (p < Cs) {
+        const int b = p / B;
+        const int sz = subtree_size(first_node + b, n);
+        if (p - b * B < sz) {
+          const int d = EXPLICIT ? (int)s_bdim[b] : rr_dim;
+          kk = ((uint64_t)ordered_key(load_coord_bits(sdata + (size_t)slot[e] * L.stride, L, d), isf) << 32) | (uint64_t)p;
+        }
       }
-      s_key[p] = k;
+      key[e] = kk;
     }
-    __syncthreads();
-    if (B > 1) bitonic_blocks<NT>(s_key, Cs, B);
-    for (int q = threadIdx.x; q < Cs; q += NT) {
-      const uint64_t k = s_key[q];
-      if (k == ~0ull) continue;
+    if (B > 1) BlockSort<CAP, NT>::sort(key, slot, B, s_x, s_xv);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      if (key[e] == ~0ull) continue;
+      const int q = p0 + e;
       const int b = q / B;
       const int node = first_node + b;
       const int r = q - b * B;
       const int left = subtree_size(2 * node + 1, n);
-      const int sl = s_slot[cur][(int)(uint32_t)k];
+      const uint32_t sl = slot[e];
       if (r == left) {
         const int d = EXPLICIT ? (int)s_bdim[b] : rr_dim;
         copy_elem(out + (size_t)node * L.stride, sdata + (size_t)sl * L.stride, L, d);
@@ -607,14 +684,15 @@ __global__ __launch_bounds__(NT) void k_finish(
         }
       } else {
         const int np = r < left ? b * B + r : b * B + (B >> 1) + (r - left - 1);
-        s_slot[cur ^ 1][np] = (uint16_t)sl;
+        s_xv[np] = sl;
       }
     }
     __syncthreads();
-    cur ^= 1;
+#pragma unroll
+    for (int e = 0; e < E; e++) slot[e] = s_xv[p0 + e];
+    __syncthreads();
   }
 }
-
 
 // ---------------------------------------------------------------------------
 // host orchestration
@@ -766,13 +844,18 @@ int build_tree_impl(void *d_points, int n, const cukd_layout &L,
     hipLaunchKernelGGL(k_level_init, dim3(((1 << level) + 255) / 256), dim3(256), 0, s,
                        sc, cursors, need, n, level, D, expl ? 1 : 0);
     const int tiles = (g.n_elems + P.T - 1) / P.T;
+    // select tiles: larger than the partition's (fewer, fatter histogram
+    // flushes), still at most two subtrees per tile (T <= F/2 <= min size)
+    int Tsel = 16384;
+    while (Tsel > g.F / 2) Tsel >>= 1;
+    const int stiles = (g.n_elems + Tsel - 1) / Tsel;
     const int max_words = expl ? D + 1 : (level + 1 < D ? level + 1 : D) + 1;
     for (int w = 0; w < max_words; w++) {
       const int shifts[3] = {21, 10, 0};
       const int bitsv[3] = {11, 11, 10};
       for (int d = 0; d < 3; d++)
-        hipLaunchKernelGGL(k_select_pass, dim3(tiles), dim3(SEL_NT), 0, s, g, w, shifts[d],
-                           bitsv[d], P.T, (const uint32_t *)kcur, (const uint8_t *)dcur,
+        hipLaunchKernelGGL(k_select_pass, dim3(stiles), dim3(SEL_NT), 0, s, g, w, shifts[d],
+                           bitsv[d], Tsel, (const uint32_t *)kcur, (const uint8_t *)dcur,
                            (const uint32_t *)icur, sc, sn, hist, need, L, expl ? 1 : 0);
     }
     hipLaunchKernelGGL(k_partition, dim3(tiles), dim3(SEL_NT), 0, s, g, gn, P.T,

@@ -56,6 +56,9 @@ size_t build_temp_bytes(int n, const cukd_layout &L);
 int build_bounds_only(void *d_bounds, const void *d_points, int n, const cukd_layout &L,
                       hipStream_t s);
 
+size_t query_order_temp_bytes(int nq);
+int32_t *make_query_order(void *tmp, const float *q, int nq, int D, hipStream_t s);
+
 hipError_t launch_query_1(int, const QueryArgs &, hipStream_t);
 hipError_t launch_query_2(int, const QueryArgs &, hipStream_t);
 hipError_t launch_query_3(int, const QueryArgs &, hipStream_t);
@@ -237,7 +240,10 @@ int cukd_fcp(int traversal, const void *d_tree, int32_t n, const cukd_layout *la
                                   nq, d_ids))
     return rc;
   if (nq == 0) return CUKD_OK;
-  (void)flags; (void)mem;
+  MemResource mr{mem};
+  void *otmp = nullptr;
+  if ((flags & CUKD_QUERY_SORT) && mr.alloc(&otmp, query_order_temp_bytes(nq), stream) != 0)
+    return CUKD_ERR_ALLOC;
   QueryArgs a;
   a.t = make_view(d_tree, n, layout);
   a.queries = d_queries;
@@ -247,12 +253,13 @@ int cukd_fcp(int traversal, const void *d_tree, int32_t n, const cukd_layout *la
   a.eps = eps;
   a.k = 1;
   a.list_kind = -1;
-  a.order = nullptr;
+  a.order = otmp ? make_query_order(otmp, d_queries, nq, layout->num_dims, stream) : nullptr;
   a.ids = d_ids;
   a.dist2 = d_dist2;
   a.ret = nullptr;
   a.stats = g_stats;
   const hipError_t e = launch_query(layout->num_dims, traversal, a, stream);
+  if (otmp) mr.release(otmp, stream);
   if (e != hipSuccess) {
     set_last_error("fcp launch failed: %s", hipGetErrorString(e));
     return CUKD_ERR_HIP;
@@ -276,7 +283,10 @@ int cukd_knn(int traversal, int list_kind, int k, const void *d_tree, int32_t n,
   }
   if (nq > 0 && !d_dist2) { set_last_error("d_dist2 is null"); return CUKD_ERR_INVALID; }
   if (nq == 0) return CUKD_OK;
-  (void)flags; (void)mem;
+  MemResource mr{mem};
+  void *otmp = nullptr;
+  if ((flags & CUKD_QUERY_SORT) && mr.alloc(&otmp, query_order_temp_bytes(nq), stream) != 0)
+    return CUKD_ERR_ALLOC;
   QueryArgs a;
   a.t = make_view(d_tree, n, layout);
   a.queries = d_queries;
@@ -286,12 +296,13 @@ int cukd_knn(int traversal, int list_kind, int k, const void *d_tree, int32_t n,
   a.eps = 0.f;
   a.k = k;
   a.list_kind = list_kind;
-  a.order = nullptr;
+  a.order = otmp ? make_query_order(otmp, d_queries, nq, layout->num_dims, stream) : nullptr;
   a.ids = d_ids;
   a.dist2 = d_dist2;
   a.ret = d_max_radius2;
   a.stats = g_stats;
   const hipError_t e = launch_query(layout->num_dims, traversal, a, stream);
+  if (otmp) mr.release(otmp, stream);
   if (e != hipSuccess) {
     set_last_error("knn launch failed: %s", hipGetErrorString(e));
     return CUKD_ERR_HIP;

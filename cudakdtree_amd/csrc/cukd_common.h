@@ -161,6 +161,12 @@ void set_last_error(const char *fmt, ...);
     }                                                                        \
   } while (0)
 
+// Exclusive prefix sum of n uint32 (stream-ordered, several launches);
+// 'scratch' holds at least scan_scratch_words(n) words.
+size_t scan_scratch_words(long long n);
+void scan_exclusive(const uint32_t *in, uint32_t *out, long long n, uint32_t *scratch,
+                    hipStream_t s);
+
 // Memory resource used for builder temporaries.
 struct MemResource {
   const cukd_mem_resource *user;

@@ -22,7 +22,7 @@
 #pragma clang fp contract(off)
 #include <new>
 
-#include "cukd/knn.h"
+#include "cukd/spatial-kdtree.h"
 #include "cukd_common.h"
 #include "query_impl.h"
 
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_add(uint32_t *out, const uint3
 }
 
 // scratch: at least scan_scratch_words(n) uint32
-static size_t scan_scratch_words(long long n)
+size_t scan_scratch_words(long long n)
 {
   size_t w = 0;
   while (n > 1) {
@@ -114,8 +114,8 @@ static size_t scan_scratch_words(long long n)
   return w + 1;
 }
 
-static void scan_exclusive(const uint32_t *in, uint32_t *out, long long n, uint32_t *scratch,
-                           hipStream_t s)
+void scan_exclusive(const uint32_t *in, uint32_t *out, long long n, uint32_t *scratch,
+                    hipStream_t s)
 {
   if (n <= 0) return;
   const long long tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -485,85 +485,6 @@ int spatial_build_impl(cukd_spatial_tree *tree, const void *d_points, int n, con
 // spatial queries
 // ---------------------------------------------------------------------------
 
-// stackBased / cct over a spatial tree; returns false on stack overflow
-template <int D, bool CCT, class R>
-__device__ inline bool spatial_traverse(R &result, const float (&q)[D],
-                                        const cukd_spatial_node *nodes,
-                                        const uint32_t *prim_ids, const float *bounds,
-                                        const TreeView &data)
-{
-  enum { DEPTH = 50 };
-  float cull = result.initialCullDist2();
-  int st_node[DEPTH];
-  float st_d2[CCT ? 1 : DEPTH];
-  float st_corner[CCT ? DEPTH : 1][D];
-  int sp = 0;
-  int node_id = 0;
-  float closest[D];
-  if (CCT) {
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      float v = q[d];
-      v = v < bounds[d] ? bounds[d] : v;
-      v = bounds[D + d] < v ? bounds[D + d] : v;
-      closest[d] = v;
-    }
-    if (cukd::detail::sqr_dist<D>(q, closest) > cull) return true;
-  }
-  for (;;) {
-    cukd_spatial_node node;
-    for (;;) {
-      node = nodes[node_id];
-      // an empty root leaf (no points) has count 0 and offset ~0u
-      if (node.count || node.offset == 0xffffffffu) break;
-      const int dim = node.dim;
-      const float qc = cukd::detail::pick<D>(q, dim);
-      const bool left_close = qc < node.pos;
-      const int lchild = (int)node.offset, rchild = (int)node.offset + 1;
-      const int far_c = left_close ? rchild : lchild;
-      if (CCT) {
-        float corner[D];
-#pragma unroll
-        for (int d = 0; d < D; d++) corner[d] = d == dim ? node.pos : closest[d];
-        if (cukd::detail::sqr_dist<D>(corner, q) < cull) {
-#pragma unroll
-          for (int d = 0; d < D; d++) st_corner[CCT ? sp : 0][d] = corner[d];
-          st_node[sp] = far_c;
-          if (++sp >= DEPTH) return false;
-        }
-      } else {
-        const float diff = qc - node.pos;
-        const float plane2 = diff * diff;
-        if (plane2 < cull) {
-          st_node[sp] = far_c;
-          st_d2[CCT ? 0 : sp] = plane2;
-          if (++sp >= DEPTH) return false;
-        }
-      }
-      node_id = left_close ? lchild : rchild;
-    }
-    for (int i = 0; i < node.count; i++) {
-      const int prim = (int)prim_ids[node.offset + i];
-      float p[D];
-      data.point<D>(prim, p);
-      cull = result.processCandidate(prim, cukd::detail::sqr_dist<D>(p, q));
-    }
-    for (;;) {
-      if (sp == 0) return true;
-      --sp;
-      if (CCT) {
-#pragma unroll
-        for (int d = 0; d < D; d++) closest[d] = st_corner[CCT ? sp : 0][d];
-        if (cukd::detail::sqr_dist<D>(closest, q) >= cull) continue;
-      } else if (st_d2[CCT ? 0 : sp] >= cull) {
-        continue;
-      }
-      node_id = st_node[sp];
-      break;
-    }
-  }
-}
-
 template <int D, bool CCT, class R>
 __global__ __launch_bounds__(128) void k_spatial_query(const cukd_spatial_node *nodes,
                                                        const uint32_t *prim_ids, TreeView data,
@@ -578,7 +499,10 @@ __global__ __launch_bounds__(128) void k_spatial_query(const cukd_spatial_node *
   for (int d = 0; d < D; d++) q[d] = queries[(size_t)i * D + d];
   typename ResultOps<R>::State st;
   R &res = ResultOps<R>::init(st, cutoff, k);
-  const bool ok = spatial_traverse<D, CCT>(res, q, nodes, prim_ids, bounds.v, data);
+  float lo[D], hi[D];
+#pragma unroll
+  for (int d = 0; d < D; d++) { lo[d] = bounds.v[d]; hi[d] = bounds.v[D + d]; }
+  const bool ok = cukd::detail::spatial_traverse<D, CCT>(res, q, nodes, prim_ids, lo, hi, data);
   ResultOps<R>::write(st, i, k, ids, dist2, ret);
   // stack overflow: fcp reports no result, knn returns -1 (fcp.h:317-320,
   // knn.h:620-623 print "STACK OVERFLOW" and return -1)

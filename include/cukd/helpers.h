@@ -30,13 +30,13 @@ struct AsyncGpuMemoryResource final : GpuMemoryResource {
 struct ManagedMemMemoryResource : GpuMemoryResource {
   hipError_t malloc(void **ptr, size_t size, hipStream_t s) override
   {
-    hipStreamSynchronize(s);
-    return hipMallocManaged(ptr, size);
+    const hipError_t e = hipStreamSynchronize(s);
+    return e != hipSuccess ? e : hipMallocManaged(ptr, size);
   }
   hipError_t free(void *ptr, hipStream_t s) override
   {
-    hipStreamSynchronize(s);
-    return hipFree(ptr);
+    const hipError_t e = hipStreamSynchronize(s);
+    return e != hipSuccess ? e : hipFree(ptr);
   }
 };
 

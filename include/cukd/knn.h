@@ -52,6 +52,10 @@ struct FixedCandidateList : public CandidateList<k> {
   inline __both__ void push(float dist, int pointID)
   {
     uint64_t v = BaseCandidateList::encode(dist, pointID);
+    // A key at or above the current k-th one would bubble through every
+    // slot and fall off the end: skipping the network changes nothing
+    // (keys are unique, ids being distinct).
+    if (v >= entry[k - 1]) return;
 #pragma unroll
     for (int i = 0; i < k; i++) {
       const uint64_t lo = entry[i] < v ? entry[i] : v;

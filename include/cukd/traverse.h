@@ -51,16 +51,21 @@ inline __both__ int round_robin_dim(int node)
   return (D & (D - 1)) == 0 ? (l & (D - 1)) : l % D;
 }
 
-// closest-point-first DFS with a 30-entry stack of (far child, plane dist2)
+// Closest-point-first DFS (the reference keeps a 30-entry stack of
+// (far child, plane dist2)).  The stack is always ordered by depth -- every
+// push while descending is deeper than the previous one, and after popping
+// depth d only deeper entries are pushed -- so it is a set of depths: bit d
+// of 'pending' marks that the far child at depth d, the sibling of the
+// current path's node at that depth, is still to be visited.  Its plane
+// distance is recomputed from its parent on pop (the same float operations
+// as at push time), so the visit order and results equal the reference's.
 template <int D, class Acc, class R>
 inline __both__ void traverse_stack_based(R &result, const float (&q)[D], const Acc &acc,
                                           unsigned &visits)
 {
   const int n = acc.size();
   float cull = result.initialCullDist2();
-  int st_node[30];
-  float st_d2[30];
-  int sp = 0;
+  unsigned pending = 0;
   int curr = 0;
   for (;;) {
     while (curr < n) {
@@ -75,19 +80,27 @@ inline __both__ void traverse_stack_based(R &result, const float (&q)[D], const 
       const int l = 2 * curr + 1;
       const int far_c = left_close ? l + 1 : l;
       const float diff = qc - nc;
-      const float plane2 = diff * diff;
-      if (plane2 < cull && far_c < n) {
-        st_node[sp] = far_c;
-        st_d2[sp] = plane2;
-        sp++;
-      }
+      if (diff * diff < cull && far_c < n) pending |= 1u << (BinaryTree::levelOf(curr) + 1);
       curr = left_close ? l : l + 1;
     }
     for (;;) {
-      if (sp == 0) return;
-      --sp;
-      if (st_d2[sp] >= cull) continue;
-      curr = st_node[sp];
+      if (!pending) return;
+#ifdef __HIP_DEVICE_COMPILE__
+      const int d = 31 - __clz((int)pending);
+#else
+      const int d = 31 - __builtin_clz(pending);
+#endif
+      pending &= ~(1u << d);
+      const int lc = BinaryTree::levelOf(curr);
+      const int on_path = ((curr + 1) >> (lc - d)) - 1;
+      const int far_c = (on_path & 1) ? on_path + 1 : on_path - 1;
+      const int parent = (far_c - 1) >> 1;
+      float p[D];
+      acc.template point<D>(parent, p);
+      const int dim = acc.dim(parent);
+      const float diff = pick<D>(q, dim) - pick<D>(p, dim);
+      if (diff * diff >= cull) continue;
+      curr = far_c;
       break;
     }
   }

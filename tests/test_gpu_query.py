@@ -140,3 +140,15 @@ def test_bench_shape_subset():
     q = datagen.uniform(20000, 3, 2)
     check_knn(*s, q, 8, float("inf"), lists=(ck.LIST_FIXED,))
     check_fcp(*s, q, float("inf"))
+
+
+@pytest.mark.parametrize("D", [2, 3, 4])
+def test_morton_ordered_queries_identical(D):
+    """CUKD_QUERY_SORT only changes the launch order: results equal the
+    oracle's and land at each query's own index."""
+    coords = distribution("gmm", 50000, D, seed=21)
+    s = _setup(coords)
+    q = datagen.uniform(20000, D, 22)
+    check_fcp(*s, q, float("inf"), flags=ck.QUERY_SORT)
+    check_knn(*s, q, 8, float("inf"), flags=ck.QUERY_SORT)
+    check_knn(*s, q, 5, 0.05, lists=(ck.LIST_HEAP,), flags=ck.QUERY_SORT)
