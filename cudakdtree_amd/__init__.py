@@ -42,6 +42,9 @@ EXPORTED_SYMBOLS = (
     "cukd_build_tree", "cukd_build_tree_temp_bytes", "cukd_compute_bounds",
     "cukd_fcp", "cukd_knn", "cukd_spatial_build", "cukd_spatial_free",
     "cukd_spatial_fcp", "cukd_spatial_knn", "cukd_set_traversal_stats",
+    "cukd_knn_accumulate", "cukd_knn_extract",
+    "cukd_forest_create", "cukd_forest_workspace_bytes", "cukd_forest_build",
+    "cukd_forest_destroy", "cukd_forest_fcp", "cukd_forest_knn",
     "cukd_last_error", "cukd_version",
 )
 
@@ -145,6 +148,16 @@ def lib():
     L.cukd_spatial_fcp.argtypes = [ctypes.c_int, SP, LP, P, i32, f32, P, P, P]
     L.cukd_spatial_knn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, SP, LP, P,
                                    i32, f32, P, P, P, P]
+    L.cukd_knn_accumulate.argtypes = [ctypes.c_int, ctypes.c_int, P, i32, LP, P, P, i32, P,
+                                       ctypes.c_int, f32, i32, ctypes.c_int, P, P]
+    L.cukd_knn_extract.argtypes = [P, i32, ctypes.c_int, P, P, P, P]
+    L.cukd_forest_create.argtypes = [ctypes.POINTER(P), P, i32, LP, P, P]
+    L.cukd_forest_workspace_bytes.argtypes = [P, ctypes.POINTER(ctypes.c_size_t)]
+    L.cukd_forest_build.argtypes = [P, P, P, P]
+    L.cukd_forest_destroy.argtypes = [P, P]
+    L.cukd_forest_fcp.argtypes = [ctypes.c_int, P, P, P, P, i32, i32, P, P, f32, f32, P, P, P]
+    L.cukd_forest_knn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, P, i32, i32,
+                                  P, P, f32, P, P, P, P]
     L.cukd_set_traversal_stats.argtypes = [P]
     L.cukd_last_error.restype = ctypes.c_char_p
     L.cukd_last_error.argtypes = []
@@ -271,6 +284,113 @@ def knn(tree, layout, queries, k, traversal=STACK_BASED, list_kind=LIST_FIXED,
                           _ptr(world_bounds), _ptr(queries), nq, cutoff, _ptr(ids),
                           _ptr(d2), _ptr(ret), flags, _stream(stream), None))
     return ids, d2, ret
+
+
+def knn_lists(nq, k, device="cuda"):
+    """Persistent per-query candidate lists for knn_accumulate (uint64 as
+    int64 storage, nq x k)."""
+    import torch
+    return torch.empty((nq, k), dtype=torch.int64, device=device)
+
+
+def knn_accumulate(tree, layout, queries, lists, init, traversal=STACK_FREE,
+                   cutoff=math.inf, id_offset=0, world_bounds=None, flags=0, stream=None):
+    """One round of a multi-round kNN (samples/mpiHugeQuery.cu:75-88): each
+    query continues its FlexHeapCandidateList in ``lists`` (nq x k)."""
+    n = _rows(tree, layout)
+    nq, k = lists.shape
+    assert queries.shape[0] == nq
+    _check(lib().cukd_knn_accumulate(traversal, k, _ptr(tree), n, ctypes.byref(layout),
+                                     _ptr(world_bounds), _ptr(queries), nq, _ptr(lists),
+                                     1 if init else 0, cutoff, id_offset, flags,
+                                     _stream(stream), None))
+    return lists
+
+
+def knn_extract(lists, stream=None):
+    """extractFinalResult (samples/mpiHugeQuery.cu:90-103): returns
+    (ids int32[nq,k] and dist2 float32[nq,k] ascending, radius float32[nq])."""
+    import torch
+    nq, k = lists.shape
+    ids = torch.empty((nq, k), dtype=torch.int32, device=lists.device)
+    d2 = torch.empty((nq, k), dtype=torch.float32, device=lists.device)
+    rad = torch.empty(nq, dtype=torch.float32, device=lists.device)
+    _check(lib().cukd_knn_extract(_ptr(lists), nq, k, _ptr(ids), _ptr(d2), _ptr(rad),
+                                  _stream(stream)))
+    return ids, d2, rad
+
+
+class Forest:
+    """Many independent trees in one array, built and queried together
+    (cukd_forest_*).  ``offsets``: num_trees+1 element offsets (host)."""
+
+    def __init__(self, offsets, layout, stream=None):
+        import numpy as np
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.offsets = off
+        self.num_trees = len(off) - 1
+        self.layout = layout
+        self._h = ctypes.c_void_p()
+        _check(lib().cukd_forest_create(ctypes.byref(self._h), off.ctypes.data, self.num_trees,
+                                        ctypes.byref(layout), _stream(stream), None))
+
+    @property
+    def workspace_bytes(self):
+        out = ctypes.c_size_t(0)
+        _check(lib().cukd_forest_workspace_bytes(self._h, ctypes.byref(out)))
+        return out.value
+
+    def bounds_tensor(self, device="cuda"):
+        import torch
+        dt = torch.float32 if self.layout.scalar_type == SCALAR_F32 else torch.int32
+        return torch.empty((self.num_trees, 2 * self.layout.num_dims), dtype=dt, device=device)
+
+    def build(self, points, world_bounds=None, stream=None):
+        _check(lib().cukd_forest_build(self._h, _ptr(points), _ptr(world_bounds),
+                                       _stream(stream)))
+        return points
+
+    def fcp(self, points, queries, seg_offsets, seg_tree, traversal=STACK_BASED,
+            cutoff=math.inf, eps=0.0, world_bounds=None, stream=None, out=None):
+        import torch
+        nq = queries.shape[0]
+        if out is None:
+            ids = torch.empty(nq, dtype=torch.int32, device=queries.device)
+            d2 = torch.empty(nq, dtype=torch.float32, device=queries.device)
+        else:
+            ids, d2 = out
+        _check(lib().cukd_forest_fcp(traversal, self._h, _ptr(points), _ptr(world_bounds),
+                                     _ptr(queries), nq, len(seg_tree), _ptr(seg_offsets),
+                                     _ptr(seg_tree), cutoff, eps, _ptr(ids), _ptr(d2),
+                                     _stream(stream)))
+        return ids, d2
+
+    def knn(self, points, queries, k, seg_offsets, seg_tree, traversal=STACK_BASED,
+            list_kind=LIST_FIXED, cutoff=math.inf, world_bounds=None, stream=None, out=None):
+        import torch
+        nq = queries.shape[0]
+        if out is None:
+            ids = torch.empty((nq, k), dtype=torch.int32, device=queries.device)
+            d2 = torch.empty((nq, k), dtype=torch.float32, device=queries.device)
+            ret = torch.empty(nq, dtype=torch.float32, device=queries.device)
+        else:
+            ids, d2, ret = out
+        _check(lib().cukd_forest_knn(traversal, list_kind, k, self._h, _ptr(points),
+                                     _ptr(world_bounds), _ptr(queries), nq, len(seg_tree),
+                                     _ptr(seg_offsets), _ptr(seg_tree), cutoff, _ptr(ids),
+                                     _ptr(d2), _ptr(ret), _stream(stream)))
+        return ids, d2, ret
+
+    def close(self, stream=None):
+        if self._h:
+            _check(lib().cukd_forest_destroy(self._h, _stream(stream)))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def set_traversal_stats(counter):

@@ -29,6 +29,11 @@
 #include <stdio.h>
 
 #include "cukd_common.h"
+#include "block_sort.h"
+#include "forest_plan.h"
+
+#include <string.h>
+#include <vector>
 
 namespace cukd_amd {
 
@@ -90,6 +95,39 @@ __device__ inline void child_chain(uint32_t parent_chain, uint32_t parent_len,
 }
 
 // ---------------------------------------------------------------------------
+// Forests: the kernels below build any number of independent trees stored
+// back to back in one array in the same launches.  A workgroup finds its
+// tree by binary search in a small table; a single tree is passed inline,
+// without a table.
+// ---------------------------------------------------------------------------
+
+template <class E>
+struct TreeSet {
+  const E *tab;  // sorted by the searched field; null: 'one' is the only tree
+  int count;
+  E one;
+};
+
+// the entry with the largest key(entry) <= v
+template <class E, class K>
+__device__ inline E find_tree(const TreeSet<E> &set, int v, K key)
+{
+  if (!set.tab) return set.one;
+  int lo = 0, hi = set.count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (key(set.tab[mid]) <= v) lo = mid; else hi = mid - 1;
+  }
+  return set.tab[lo];
+}
+
+template <class E>
+__device__ inline E tree_at(const TreeSet<E> &set, int i)
+{
+  return set.tab ? set.tab[i] : set.one;
+}
+
+// ---------------------------------------------------------------------------
 // zero fill (used instead of hipMemsetAsync so the sequence can be captured)
 // ---------------------------------------------------------------------------
 __global__ void k_zero(uint32_t *p, size_t n)
@@ -103,23 +141,31 @@ __global__ void k_zero(uint32_t *p, size_t n)
 // world bounds (builder_common.h:76-163): block min/max of order-preserving
 // keys, one atomic per dimension and block.
 // ---------------------------------------------------------------------------
-__global__ void k_bounds_init(uint32_t *acc, int D)
+__global__ void k_bounds_init(uint32_t *acc, int D, int ntrees)
 {
-  const int d = threadIdx.x;
-  if (d < D) { acc[d] = 0xffffffffu; acc[D + d] = 0u; }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ntrees * 2 * D) acc[i] = (i % (2 * D)) < D ? 0xffffffffu : 0u;
 }
 
-__global__ __launch_bounds__(256) void k_bounds(uint32_t *acc,
-                                                const uint8_t *points, int n,
-                                                cukd_layout L)
+// one workgroup per IN_TILE elements of one tree
+__global__ __launch_bounds__(256) void k_bounds(TreeSet<InTree> set, uint32_t *acc,
+                                                const uint8_t *points, cukd_layout L)
 {
   __shared__ uint32_t s_lo[CUKD_MAX_DIMS][256], s_hi[CUKD_MAX_DIMS][256];
+  __shared__ InTree s_tree;
+  if (threadIdx.x == 0)
+    s_tree = find_tree(set, (int)blockIdx.x, [](const InTree &x) { return x.tile0; });
+  __syncthreads();
+  const InTree tr = s_tree;
   const int D = L.num_dims;
   const bool isf = L.scalar_type == CUKD_SCALAR_F32;
+  const int e0 = ((int)blockIdx.x - tr.tile0) * IN_TILE;
+  const int e1 = min(e0 + IN_TILE, tr.n);
+  const uint8_t *pts = points + (size_t)tr.base * L.stride;
   uint32_t lo[CUKD_MAX_DIMS], hi[CUKD_MAX_DIMS];
   for (int d = 0; d < D; d++) { lo[d] = 0xffffffffu; hi[d] = 0u; }
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint8_t *e = points + (size_t)i * L.stride;
+  for (int i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
+    const uint8_t *e = pts + (size_t)i * L.stride;
     for (int d = 0; d < D; d++) {
       const uint32_t k = ordered_key(load_coord_bits(e, L, d), isf);
       lo[d] = min(lo[d], k);
@@ -137,53 +183,65 @@ __global__ __launch_bounds__(256) void k_bounds(uint32_t *acc,
     __syncthreads();
   }
   if ((int)threadIdx.x < D) {
-    atomicMin(&acc[threadIdx.x], s_lo[threadIdx.x][0]);
-    atomicMax(&acc[D + threadIdx.x], s_hi[threadIdx.x][0]);
+    uint32_t *a = acc + (size_t)tr.id * 2 * D;
+    atomicMin(&a[threadIdx.x], s_lo[threadIdx.x][0]);
+    atomicMax(&a[D + threadIdx.x], s_hi[threadIdx.x][0]);
   }
 }
 
-// Decodes the accumulated bounds into the caller's box and seeds the root
-// subtree's state (domain = world box, dim = widest, builder_thrust.h:
-// 144-156).
-__global__ void k_bounds_final(const uint32_t *acc, uint32_t *user_box,
-                               SegState *root, cukd_layout L, int explicit_dim)
+// Decodes every tree's accumulated bounds into the caller's boxes and
+// seeds its root state (domain = world box, dim = widest,
+// builder_thrust.h:144-156).
+__global__ void k_bounds_final(TreeSet<InTree> set, int ntrees, const uint32_t *acc,
+                               uint32_t *user_box, SegState *st, cukd_layout L,
+                               int explicit_dim)
 {
-  if (threadIdx.x != 0) return;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ntrees) return;
+  const InTree tr = tree_at(set, j);
   const int D = L.num_dims;
   const bool isf = L.scalar_type == CUKD_SCALAR_F32;
   uint32_t box[2 * CUKD_MAX_DIMS];
-  for (int i = 0; i < 2 * D; i++) box[i] = unordered_key(acc[i], isf);
+  for (int i = 0; i < 2 * D; i++) box[i] = unordered_key(acc[(size_t)tr.id * 2 * D + i], isf);
   if (user_box)
-    for (int i = 0; i < 2 * D; i++) user_box[i] = box[i];
-  if (root) {
-    for (int i = 0; i < 2 * D; i++) root->box[i] = box[i];
+    for (int i = 0; i < 2 * D; i++) user_box[(size_t)tr.id * 2 * D + i] = box[i];
+  if (st) {
+    SegState &root = st[tr.seg];
+    for (int i = 0; i < 2 * D; i++) root.box[i] = box[i];
     const int d0 = explicit_dim ? widest_dim_bits(box, D, isf) : 0;
-    root->dim = (uint32_t)d0;
-    root->chain = (uint32_t)d0;
-    root->chain_len = 1;
+    root.dim = (uint32_t)d0;
+    root.chain = (uint32_t)d0;
+    root.chain_len = 1;
   }
 }
 
 // ---------------------------------------------------------------------------
-// copy-in: working buffer A gets the points, their input index, and the
-// level-0 key.
+// copy-in: working buffer A gets the points, their input index inside their
+// tree, and the level-0 key.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_copy_in(uint8_t *data, uint32_t *idx,
-                                                 uint32_t *key,
-                                                 const uint8_t *points, int n,
-                                                 cukd_layout L,
-                                                 const SegState *root,
-                                                 int explicit_dim)
+__global__ __launch_bounds__(256) void k_copy_in(TreeSet<InTree> set, uint8_t *data,
+                                                 uint32_t *idx, uint32_t *key,
+                                                 const uint8_t *points, cukd_layout L,
+                                                 const SegState *st, int explicit_dim)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int d0 = explicit_dim ? (int)root->dim : 0;
-  const uint8_t *src = points + (size_t)i * L.stride;
+  __shared__ InTree s_tree;
+  if (threadIdx.x == 0)
+    s_tree = find_tree(set, (int)blockIdx.x, [](const InTree &x) { return x.tile0; });
+  __syncthreads();
+  const InTree tr = s_tree;
+  const int d0 = explicit_dim ? (int)st[tr.seg].dim : 0;
+  const int e0 = ((int)blockIdx.x - tr.tile0) * IN_TILE;
+  const int e1 = min(e0 + IN_TILE, tr.n);
   const int W = L.stride >> 2;
-  uint32_t *dst = (uint32_t *)(data + (size_t)i * L.stride);
-  for (int w = 0; w < W; w++) dst[w] = ((const uint32_t *)src)[w];
-  idx[i] = (uint32_t)i;
-  if (key) key[i] = ordered_key(load_coord_bits(src, L, d0), L.scalar_type == CUKD_SCALAR_F32);
+  const bool isf = L.scalar_type == CUKD_SCALAR_F32;
+  for (int i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
+    const size_t g = (size_t)tr.base + i;
+    const uint8_t *src = points + g * L.stride;
+    uint32_t *dst = (uint32_t *)(data + g * L.stride);
+    for (int w = 0; w < W; w++) dst[w] = ((const uint32_t *)src)[w];
+    idx[g] = (uint32_t)i;
+    if (key) key[g] = ordered_key(load_coord_bits(src, L, d0), isf);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -191,17 +249,20 @@ __global__ __launch_bounds__(256) void k_copy_in(uint8_t *data, uint32_t *idx,
 // ---------------------------------------------------------------------------
 
 // Per-level reset of the subtree states; round-robin dims and chains are
-// closed forms of the level.
-__global__ void k_level_init(SegState *st, uint32_t *cursors, uint32_t *need,
-                             int n, int level, int D, int explicit_dim)
+// closed forms of the level.  One thread per segment of the level.
+__global__ void k_level_init(TreeSet<LvlTree> set, int nsegs, SegState *st,
+                             uint32_t *cursors, uint32_t *need, int level, int D,
+                             int explicit_dim)
 {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s == 0)
+  const int sg = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sg == 0)
     for (int w = 0; w < MAX_WORDS; w++) need[w] = 0u;
-  if (s >= (1 << level)) return;
+  if (sg >= nsegs) return;
+  const LvlTree tr = find_tree(set, sg, [](const LvlTree &x) { return x.hseg; });
+  const int s = sg - tr.hseg;
   const int node = (1 << level) - 1 + s;
-  SegState &t = st[s];
-  t.krem = (uint32_t)subtree_size(2 * node + 1, n);
+  SegState &t = st[tr.seg + s];
+  t.krem = (uint32_t)subtree_size(2 * node + 1, tr.n);
   t.word = 0;
   t.prefix = 0;
   t.resolved = 0;
@@ -213,8 +274,8 @@ __global__ void k_level_init(SegState *st, uint32_t *cursors, uint32_t *need,
     t.chain = chain;
     t.chain_len = len;
   }
-  cursors[2 * s] = 0u;
-  cursors[2 * s + 1] = 0u;
+  cursors[2 * sg] = 0u;
+  cursors[2 * sg + 1] = 0u;
 }
 
 // Value of composite-key word w of an element (word 0 is the cached key).
@@ -249,10 +310,9 @@ __device__ inline int tiles_of(const LevelGeom &g, int s, int T)
 // the subtree's tile counter; the tile that sees the final count reads and
 // clears the bins with atomic exchanges.
 __global__ __launch_bounds__(SEL_NT) void k_select_pass(
-    LevelGeom g, int word, int shift, int bits, int T,
+    TreeSet<LvlTree> set, int level, int word, int shift, int bits,
     const uint32_t *key, const uint8_t *data, const uint32_t *idx,
-    SegState *st, SegState *child_st, uint32_t *hist, uint32_t *need,
-    cukd_layout L, int explicit_dim)
+    SegState *st, uint32_t *hist, uint32_t *need, cukd_layout L, int explicit_dim)
 {
   if (word > 0 && __hip_atomic_load(&need[word], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
     return;
@@ -261,9 +321,23 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
   __shared__ int s_seg[2];
   __shared__ int s_last[2];
   __shared__ uint32_t s_sink;
+  __shared__ LvlTree s_tree;
+  if (threadIdx.x == 0)
+    s_tree = find_tree(set, (int)blockIdx.x, [](const LvlTree &x) { return x.stile0; });
+  __syncthreads();
+  const LvlTree tr = s_tree;
+  LevelGeom g;
+  g.init(tr.n, level);
+  const int T = tr.tsel;
+  key += tr.base;
+  data += (size_t)tr.base * L.stride;
+  idx += tr.base;
+  SegState *child_st = st + tr.cseg;
+  st += tr.seg;
+  hist += (size_t)tr.hseg * DIGIT_MAX_BINS;
   const bool isf = L.scalar_type == CUKD_SCALAR_F32;
   const int nbins = 1 << bits;
-  const int t0 = blockIdx.x * T;
+  const int t0 = ((int)blockIdx.x - tr.stile0) * T;
   const int t1 = min(t0 + T, g.n_elems);
   const int seg0 = g.seg_of(t0);
   const int seg1 = g.seg_of(t1 - 1);
@@ -380,19 +454,37 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
 // level's compact layout (order inside a child is irrelevant: the next
 // level selects by explicit key).
 __global__ __launch_bounds__(SEL_NT) void k_partition(
-    LevelGeom g, LevelGeom gn, int T, const uint32_t *key,
-    const uint8_t *data, const uint32_t *idx, const SegState *st,
-    const SegState *child_st, uint8_t *ndata, uint32_t *nidx, uint32_t *nkey,
-    uint32_t *cursors, uint8_t *out, cukd_layout L, int explicit_dim,
-    int write_next_key)
+    TreeSet<LvlTree> set, int level, const uint32_t *key, const uint8_t *data,
+    const uint32_t *idx, const SegState *st, uint8_t *ndata, uint32_t *nidx, uint32_t *nkey,
+    uint32_t *cursors, uint8_t *out, cukd_layout L, int explicit_dim)
 {
   __shared__ SegState s_st[2];
   __shared__ uint32_t s_cdim[4];
   __shared__ uint64_t s_scan[SEL_NT];
   __shared__ uint32_t s_base[4];
+  __shared__ LvlTree s_tree;
+  if (threadIdx.x == 0)
+    s_tree = find_tree(set, (int)blockIdx.x, [](const LvlTree &x) { return x.ptile0; });
+  __syncthreads();
+  const LvlTree tr = s_tree;
+  LevelGeom g, gn;
+  g.init(tr.n, level);
+  gn.init(tr.n, level + 1);
+  const int T = tr.tpart;
+  const int write_next_key = tr.more;
+  key += tr.base;
+  data += (size_t)tr.base * L.stride;
+  idx += tr.base;
+  ndata += (size_t)tr.base * L.stride;
+  nidx += tr.base;
+  nkey += tr.base;
+  out += (size_t)tr.base * L.stride;
+  cursors += 2 * tr.hseg;
+  const SegState *child_st = st + tr.cseg;
+  st += tr.seg;
   const bool isf = L.scalar_type == CUKD_SCALAR_F32;
   const int D = L.num_dims;
-  const int t0 = blockIdx.x * T;
+  const int t0 = ((int)blockIdx.x - tr.ptile0) * T;
   const int t1 = min(t0 + T, g.n_elems);
   const int seg0 = g.seg_of(t0);
   const int seg1 = g.seg_of(t1 - 1);
@@ -474,85 +566,10 @@ __global__ __launch_bounds__(SEL_NT) void k_partition(
 // LDS finish: one workgroup builds a whole subtree of at most CAP-1 nodes.
 // ---------------------------------------------------------------------------
 
-// Register-blocked bitonic sort of CAP 64-bit keys (with a 32-bit payload)
-// held E = CAP/NT per thread, thread t owning positions t*E .. t*E+E-1;
-// sorts independent aligned blocks of size B ascending.  Strides below E
-// stay in registers, strides below 64*E use wave shuffles, larger ones go
-// through LDS.  Keys are unique apart from the ~0 padding.
-template <int CAP, int NT>
-struct BlockSort {
-  enum { E = CAP / NT };
-
-  // keep min (true) or max of (key, other) at position i
-  static __device__ inline void cmpx(uint64_t &k, uint32_t &v, uint64_t ok, uint32_t ov, bool keep_min)
-  {
-    const bool take = keep_min ? (ok < k) : (ok > k);
-    k = take ? ok : k;
-    v = take ? ov : v;
-  }
-
-  template <int J>
-  static __device__ inline void reg_stage(uint64_t (&k)[E], uint32_t (&v)[E], int k_size, int B)
-  {
-    const int base = threadIdx.x * E;
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-      if (e & J) continue;
-      const int i = base + e;
-      const bool asc = (k_size == B) || ((i & k_size) == 0);
-      const uint64_t a = k[e], b = k[e + J];
-      const bool sw = asc ? (a > b) : (a < b);
-      if (sw) {
-        k[e] = b; k[e + J] = a;
-        const uint32_t t = v[e]; v[e] = v[e + J]; v[e + J] = t;
-      }
-    }
-  }
-
-  static __device__ inline void sort(uint64_t (&k)[E], uint32_t (&v)[E], int B, uint64_t *sx,
-                                     uint32_t *sv)
-  {
-    const int base = threadIdx.x * E;
-    for (int ks = 2; ks <= B; ks <<= 1) {
-      for (int j = ks >> 1; j > 0; j >>= 1) {
-        if (j < E) {
-          if (j >= 8) { if (E > 8) reg_stage<(E > 8 ? 8 : 1)>(k, v, ks, B); }
-          else if (j == 4) { if (E > 4) reg_stage<(E > 4 ? 4 : 1)>(k, v, ks, B); }
-          else if (j == 2) { if (E > 2) reg_stage<(E > 2 ? 2 : 1)>(k, v, ks, B); }
-          else { reg_stage<1>(k, v, ks, B); }
-          continue;
-        }
-        const int lx = j / E;  // partner thread = tid ^ lx, same e
-        if (lx < 64) {
-#pragma unroll
-          for (int e = 0; e < E; e++) {
-            const uint64_t ok = __shfl_xor(k[e], lx);
-            const uint32_t ov = __shfl_xor(v[e], lx);
-            const int i = base + e;
-            const bool asc = (ks == B) || ((i & ks) == 0);
-            cmpx(k[e], v[e], ok, ov, ((i & j) == 0) == asc);
-          }
-        } else {
-#pragma unroll
-          for (int e = 0; e < E; e++) { sx[base + e] = k[e]; sv[base + e] = v[e]; }
-          __syncthreads();
-#pragma unroll
-          for (int e = 0; e < E; e++) {
-            const int i = base + e;
-            const bool asc = (ks == B) || ((i & ks) == 0);
-            cmpx(k[e], v[e], sx[i ^ j], sv[i ^ j], ((i & j) == 0) == asc);
-          }
-          __syncthreads();
-        }
-      }
-    }
-  }
-};
-
 template <int CAP, int NT, bool EXPLICIT>
 __global__ __launch_bounds__(NT) void k_finish(
-    int n, int L0, const uint8_t *data, const uint32_t *idx,
-    const SegState *st, uint8_t *out, cukd_layout L)
+    TreeSet<FinTree> set, const uint8_t *dataA, const uint8_t *dataB, const uint32_t *idxA,
+    const uint32_t *idxB, const SegState *st, uint8_t *out, cukd_layout L)
 {
   enum { E = CAP / NT };
   __shared__ uint64_t s_x[CAP];
@@ -561,13 +578,24 @@ __global__ __launch_bounds__(NT) void k_finish(
   __shared__ uint8_t s_sdim[EXPLICIT ? CAP : 1];
   __shared__ uint8_t s_bdim[EXPLICIT ? CAP / 2 : 1];
   __shared__ uint32_t s_box[2 * CUKD_MAX_DIMS];
+  __shared__ FinTree s_tree;
+  if (threadIdx.x == 0)
+    s_tree = find_tree(set, (int)blockIdx.x, [](const FinTree &x) { return x.blk0; });
+  __syncthreads();
+  const FinTree tr = s_tree;
+  const int n = tr.n, L0 = tr.L0;
+  // after L0 partitions a tree's working copy is in buffer A (L0 even) or B
+  const uint8_t *data = ((L0 & 1) ? dataB : dataA) + (size_t)tr.base * L.stride;
+  const uint32_t *idx = ((L0 & 1) ? idxB : idxA) + tr.base;
+  out += (size_t)tr.base * L.stride;
+  st += tr.seg;
 
   const bool isf = L.scalar_type == CUKD_SCALAR_F32;
   const int D = L.num_dims;
   const int h = num_levels_for(n);
   const int hs = h - L0;            // levels in this subtree
   const int Cs = 1 << hs;           // aligned capacity (<= CAP)
-  const int s = blockIdx.x;
+  const int s = (int)blockIdx.x - tr.blk0;
   LevelGeom g;
   g.init(n, L0);
   const int g_node = (1 << L0) - 1 + s;
@@ -700,74 +728,289 @@ __global__ __launch_bounds__(NT) void k_finish(
 
 namespace {
 
-struct BuildPlan {
-  int n, h, L0, cap, T;
-  size_t stride;
-  size_t off_dataA, off_idxA, off_keyA, off_dataB, off_idxB, off_keyB;
-  size_t off_st0, off_st1, off_hist, off_cursors, off_need, off_acc;
-  size_t total;
-};
-
 inline size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
-void make_plan(int n, const cukd_layout &L, BuildPlan &P)
+inline int cap_class(int cap) { return 31 - __builtin_clz((unsigned)cap) - 9; }  // 512 -> 0
+
+// LDS-finish capacity and number of top levels of one tree.  In a forest
+// with enough finish workgroups to fill the GPU every tree takes the
+// largest capacity (fewest top levels); otherwise the capacity shrinks
+// until the tree alone has >= 256 finish workgroups.
+void tree_cap(int n, bool many, int &cap, int &L0)
 {
-  P.n = n;
-  P.h = num_levels_for(n);
-  P.stride = (size_t)L.stride;
-  int cap = 8192;
-  int L0 = 0;
+  const int h = num_levels_for(n);
+  L0 = 0;
   for (cap = 8192; cap >= 512; cap >>= 1) {
     const int lg = 31 - __builtin_clz((unsigned)cap);
-    L0 = P.h - lg > 0 ? P.h - lg : 0;
-    if ((1 << L0) >= 256 || cap == 512) break;
+    L0 = h - lg > 0 ? h - lg : 0;
+    if (many || (1 << L0) >= 256 || cap == 512) break;
   }
   if (L0 == 0) {
     // whole tree in one workgroup: capacity = 2^h
     cap = 512;
-    while (cap < (1 << P.h)) cap <<= 1;
+    while (cap < (1 << h)) cap <<= 1;
   }
-  P.L0 = L0;
-  P.cap = cap;
-  P.T = cap / 2 < SEL_NT * SEL_ITEMS ? cap / 2 : SEL_NT * SEL_ITEMS;
-  size_t o = 0;
-  const size_t nn = (size_t)n;
-  P.off_dataA = o; o = align_up(o + nn * P.stride);
-  P.off_idxA = o; o = align_up(o + nn * 4);
-  P.off_keyA = o; o = align_up(o + nn * 4);
-  if (L0 > 0) {
-    P.off_dataB = o; o = align_up(o + nn * P.stride);
-    P.off_idxB = o; o = align_up(o + nn * 4);
-    P.off_keyB = o; o = align_up(o + nn * 4);
-  } else {
-    P.off_dataB = P.off_idxB = P.off_keyB = 0;
-  }
-  const size_t nst = (size_t)1 << L0;
-  P.off_st0 = o; o = align_up(o + nst * sizeof(SegState));
-  P.off_st1 = o; o = align_up(o + nst * sizeof(SegState));
-  const size_t nseg_prev = L0 > 0 ? ((size_t)1 << (L0 - 1)) : 1;
-  P.off_hist = o; o = align_up(o + nseg_prev * DIGIT_MAX_BINS * 4);
-  P.off_cursors = o; o = align_up(o + nseg_prev * 2 * 4);
-  P.off_need = o; o = align_up(o + MAX_WORDS * 4);
-  P.off_acc = o; o = align_up(o + 2 * CUKD_MAX_DIMS * 4);
-  P.total = o;
 }
 
 template <int CAP, int NT>
-hipError_t launch_finish(bool expl, int blocks, hipStream_t s, int n, int L0,
-                         const uint8_t *data, const uint32_t *idx,
-                         const SegState *st, uint8_t *out, const cukd_layout &L)
+hipError_t launch_finish(bool expl, const TreeSet<FinTree> &set, int blocks, hipStream_t s,
+                         const uint8_t *dA, const uint8_t *dB, const uint32_t *iA,
+                         const uint32_t *iB, const SegState *st, uint8_t *out,
+                         const cukd_layout &L)
 {
   if (expl)
-    hipLaunchKernelGGL((k_finish<CAP, NT, true>), dim3(blocks), dim3(NT), 0, s,
-                       n, L0, data, idx, st, out, L);
+    hipLaunchKernelGGL((k_finish<CAP, NT, true>), dim3(blocks), dim3(NT), 0, s, set, dA, dB, iA,
+                       iB, st, out, L);
   else
-    hipLaunchKernelGGL((k_finish<CAP, NT, false>), dim3(blocks), dim3(NT), 0, s,
-                       n, L0, data, idx, st, out, L);
+    hipLaunchKernelGGL((k_finish<CAP, NT, false>), dim3(blocks), dim3(NT), 0, s, set, dA, dB, iA,
+                       iB, st, out, L);
   return hipGetLastError();
 }
 
 }  // namespace
+
+int make_forest_plan(ForestPlan &P, const long long *off, int ntrees, const cukd_layout &L,
+                     bool device_tables)
+{
+  P = ForestPlan();
+  P.L = L;
+  P.expl = L.dim_offset >= 0;
+  P.ntrees = ntrees;
+  P.ntotal = off[ntrees] - off[0];
+  P.tables = device_tables;
+  const int D = L.num_dims;
+  std::vector<int> cap(ntrees, 0), L0(ntrees, 0);
+  long long blocks_max_cap = 0;
+  for (int t = 0; t < ntrees; t++) {
+    const int n = (int)(off[t + 1] - off[t]);
+    if (n < 1) continue;
+    tree_cap(n, true, cap[t], L0[t]);
+    blocks_max_cap += 1ll << L0[t];
+  }
+  const bool many = blocks_max_cap >= 512;
+  P.max_L0 = 0;
+  for (int t = 0; t < ntrees; t++) {
+    const int n = (int)(off[t + 1] - off[t]);
+    if (n < 1) continue;
+    tree_cap(n, many, cap[t], L0[t]);
+    P.max_L0 = L0[t] > P.max_L0 ? L0[t] : P.max_L0;
+  }
+  // segment states, numbered level by level
+  std::vector<std::vector<int>> seg(ntrees);
+  size_t next = 0;
+  for (int l = 0; l <= P.max_L0; l++)
+    for (int t = 0; t < ntrees; t++) {
+      if (off[t + 1] - off[t] < 1 || L0[t] < l) continue;
+      seg[t].push_back((int)next);
+      next += (size_t)1 << l;
+    }
+  P.nstates = next;
+  // top levels
+  P.max_hseg = 1;
+  for (int l = 0; l < P.max_L0; l++) {
+    std::vector<LvlTree> v;
+    int hseg = 0, stl = 0, ptl = 0;
+    for (int t = 0; t < ntrees; t++) {
+      const int n = (int)(off[t + 1] - off[t]);
+      if (n < 1 || L0[t] <= l) continue;
+      LvlTree x;
+      x.n = n;
+      x.base = (int)(off[t] - off[0]);
+      x.seg = seg[t][l];
+      x.cseg = seg[t][l + 1];
+      x.hseg = hseg;
+      LevelGeom g;
+      g.init(n, l);
+      // select tiles: larger than the partition's (fewer, fatter histogram
+      // flushes), still at most two subtrees per tile (T <= F/2 <= min size)
+      int tsel = 16384;
+      while (tsel > g.F / 2) tsel >>= 1;
+      x.tsel = tsel;
+      x.tpart = cap[t] / 2 < SEL_NT * SEL_ITEMS ? cap[t] / 2 : SEL_NT * SEL_ITEMS;
+      x.stile0 = stl;
+      stl += (g.n_elems + tsel - 1) / tsel;
+      x.ptile0 = ptl;
+      ptl += (g.n_elems + x.tpart - 1) / x.tpart;
+      x.more = l + 1 < L0[t] ? 1 : 0;
+      hseg += 1 << l;
+      v.push_back(x);
+    }
+    P.lvl.push_back(v);
+    P.lvl_segs.push_back(hseg);
+    P.lvl_stiles.push_back(stl);
+    P.lvl_ptiles.push_back(ptl);
+    P.max_hseg = (size_t)hseg > P.max_hseg ? (size_t)hseg : P.max_hseg;
+  }
+  // finish workgroups, by capacity class
+  for (int c = 0; c < 5; c++) P.fin_blocks[c] = 0;
+  for (int t = 0; t < ntrees; t++) {
+    const int n = (int)(off[t + 1] - off[t]);
+    if (n < 1) continue;
+    const int c = cap_class(cap[t]);
+    FinTree f;
+    f.n = n;
+    f.base = (int)(off[t] - off[0]);
+    f.L0 = L0[t];
+    f.seg = seg[t][L0[t]];
+    f.blk0 = P.fin_blocks[c];
+    P.fin_blocks[c] += 1 << L0[t];
+    P.fin[c].push_back(f);
+  }
+  // bounds / copy-in tiles
+  P.in_tiles = 0;
+  for (int t = 0; t < ntrees; t++) {
+    const int n = (int)(off[t + 1] - off[t]);
+    if (n < 1) continue;
+    InTree x;
+    x.n = n;
+    x.base = (int)(off[t] - off[0]);
+    x.id = t;
+    x.seg = seg[t][0];
+    x.tile0 = P.in_tiles;
+    P.in_tiles += (n + IN_TILE - 1) / IN_TILE;
+    P.in.push_back(x);
+  }
+  // workspace
+  const size_t N = (size_t)P.ntotal, S = (size_t)L.stride;
+  size_t o = 0;
+  P.off_dataA = o; o = align_up(o + N * S);
+  P.off_idxA = o; o = align_up(o + N * 4);
+  P.off_keyA = o; o = align_up(o + N * 4);
+  if (P.max_L0 > 0) {
+    P.off_dataB = o; o = align_up(o + N * S);
+    P.off_idxB = o; o = align_up(o + N * 4);
+    P.off_keyB = o; o = align_up(o + N * 4);
+  } else {
+    P.off_dataB = P.off_idxB = P.off_keyB = 0;
+  }
+  P.off_st = o; o = align_up(o + P.nstates * sizeof(SegState));
+  P.off_hist = o; o = align_up(o + P.max_hseg * DIGIT_MAX_BINS * 4);
+  P.off_cursors = o; o = align_up(o + P.max_hseg * 2 * 4);
+  P.off_need = o; o = align_up(o + MAX_WORDS * 4);
+  P.zero_end = o;
+  P.off_acc = o; o = align_up(o + (size_t)ntrees * 2 * D * 4);
+  if (device_tables) {
+    P.off_tab_in = o; o = align_up(o + P.in.size() * sizeof(InTree));
+    for (int c = 0; c < 5; c++) { P.off_tab_fin[c] = o; o = align_up(o + P.fin[c].size() * sizeof(FinTree)); }
+    P.off_tab_lvl.resize(P.lvl.size());
+    for (size_t l = 0; l < P.lvl.size(); l++) {
+      P.off_tab_lvl[l] = o;
+      o = align_up(o + P.lvl[l].size() * sizeof(LvlTree));
+    }
+  }
+  P.total = o;
+  return CUKD_OK;
+}
+
+// host image of the device tables of a forest plan (offsets relative to
+// the workspace)
+void forest_tables_image(const ForestPlan &P, std::vector<uint8_t> &img, size_t &begin)
+{
+  begin = P.off_tab_in;
+  img.assign(P.total - begin, 0);
+  auto put = [&](size_t off, const void *src, size_t bytes) {
+    if (bytes) memcpy(img.data() + (off - begin), src, bytes);
+  };
+  put(P.off_tab_in, P.in.data(), P.in.size() * sizeof(InTree));
+  for (int c = 0; c < 5; c++) put(P.off_tab_fin[c], P.fin[c].data(), P.fin[c].size() * sizeof(FinTree));
+  for (size_t l = 0; l < P.lvl.size(); l++)
+    put(P.off_tab_lvl[l], P.lvl[l].data(), P.lvl[l].size() * sizeof(LvlTree));
+}
+
+template <class E>
+static TreeSet<E> make_set(const ForestPlan &P, uint8_t *ws, size_t off, const std::vector<E> &v)
+{
+  TreeSet<E> s;
+  s.count = (int)v.size();
+  if (P.tables) {
+    s.tab = (const E *)(ws + off);
+    memset(&s.one, 0, sizeof(E));
+  } else {
+    s.tab = nullptr;
+    s.one = v[0];
+  }
+  return s;
+}
+
+// Launches a whole build of plan P over d_points with workspace ws (tables
+// already in place when P.tables).  No allocation, no host synchronisation.
+int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_t *ws,
+                     hipStream_t s)
+{
+  const cukd_layout &L = P.L;
+  const bool expl = P.expl;
+  const int D = L.num_dims;
+  if (P.in.empty()) return CUKD_OK;
+  uint8_t *dataA = ws + P.off_dataA, *dataB = ws + P.off_dataB;
+  uint32_t *idxA = (uint32_t *)(ws + P.off_idxA), *idxB = (uint32_t *)(ws + P.off_idxB);
+  uint32_t *keyA = (uint32_t *)(ws + P.off_keyA), *keyB = (uint32_t *)(ws + P.off_keyB);
+  SegState *st = (SegState *)(ws + P.off_st);
+  uint32_t *hist = (uint32_t *)(ws + P.off_hist);
+  uint32_t *cursors = (uint32_t *)(ws + P.off_cursors);
+  uint32_t *need = (uint32_t *)(ws + P.off_need);
+  uint32_t *acc = (uint32_t *)(ws + P.off_acc);
+  uint8_t *pts = (uint8_t *)d_points;
+
+  // zero the state area (states, histograms, cursors, flags)
+  hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, s, (uint32_t *)(ws + P.off_st),
+                     (P.zero_end - P.off_st) / 4);
+  const TreeSet<InTree> in = make_set(P, ws, P.off_tab_in, P.in);
+  if (d_bounds || expl) {
+    const int nacc = P.ntrees * 2 * D;
+    hipLaunchKernelGGL(k_bounds_init, dim3((nacc + 255) / 256), dim3(256), 0, s, acc, D,
+                       P.ntrees);
+    hipLaunchKernelGGL(k_bounds, dim3(P.in_tiles), dim3(256), 0, s, in, acc,
+                       (const uint8_t *)pts, L);
+    hipLaunchKernelGGL(k_bounds_final, dim3(((int)P.in.size() + 255) / 256), dim3(256), 0, s, in,
+                       (int)P.in.size(), (const uint32_t *)acc, (uint32_t *)d_bounds, st, L,
+                       expl ? 1 : 0);
+  }
+  hipLaunchKernelGGL(k_copy_in, dim3(P.in_tiles), dim3(256), 0, s, in, dataA, idxA,
+                     P.max_L0 > 0 ? keyA : (uint32_t *)nullptr, (const uint8_t *)pts, L,
+                     (const SegState *)st, expl ? 1 : 0);
+  uint8_t *dcur = dataA, *dnext = dataB;
+  uint32_t *icur = idxA, *inext = idxB, *kcur = keyA, *knext = keyB;
+  for (int level = 0; level < P.max_L0; level++) {
+    const TreeSet<LvlTree> set =
+        make_set(P, ws, P.tables ? P.off_tab_lvl[level] : 0, P.lvl[level]);
+    hipLaunchKernelGGL(k_level_init, dim3((P.lvl_segs[level] + 255) / 256), dim3(256), 0, s, set,
+                       P.lvl_segs[level], st, cursors, need, level, D, expl ? 1 : 0);
+    const int max_words = expl ? D + 1 : (level + 1 < D ? level + 1 : D) + 1;
+    for (int w = 0; w < max_words; w++) {
+      const int shifts[3] = {21, 10, 0};
+      const int bitsv[3] = {11, 11, 10};
+      for (int d = 0; d < 3; d++)
+        hipLaunchKernelGGL(k_select_pass, dim3(P.lvl_stiles[level]), dim3(SEL_NT), 0, s, set,
+                           level, w, shifts[d], bitsv[d], (const uint32_t *)kcur,
+                           (const uint8_t *)dcur, (const uint32_t *)icur, st, hist, need, L,
+                           expl ? 1 : 0);
+    }
+    hipLaunchKernelGGL(k_partition, dim3(P.lvl_ptiles[level]), dim3(SEL_NT), 0, s, set, level,
+                       (const uint32_t *)kcur, (const uint8_t *)dcur, (const uint32_t *)icur,
+                       (const SegState *)st, dnext, inext, knext, cursors, pts, L, expl ? 1 : 0);
+    uint8_t *t8 = dcur; dcur = dnext; dnext = t8;
+    uint32_t *t32 = icur; icur = inext; inext = t32;
+    t32 = kcur; kcur = knext; knext = t32;
+  }
+  hipError_t e = hipGetLastError();
+  for (int c = 0; c < 5 && e == hipSuccess; c++) {
+    if (P.fin_blocks[c] == 0) continue;
+    const TreeSet<FinTree> fs = make_set(P, ws, P.tables ? P.off_tab_fin[c] : 0, P.fin[c]);
+    const int b = P.fin_blocks[c];
+    switch (c) {
+      case 0: e = launch_finish<512, 256>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
+      case 1: e = launch_finish<1024, 256>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
+      case 2: e = launch_finish<2048, 512>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
+      case 3: e = launch_finish<4096, 512>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
+      default: e = launch_finish<8192, 512>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
+    }
+  }
+  if (e != hipSuccess) {
+    set_last_error("builder kernel launch failed: %s", hipGetErrorString(e));
+    return CUKD_ERR_HIP;
+  }
+  return CUKD_OK;
+}
 
 // World bounds only (cukd_compute_bounds): the caller's box doubles as the
 // accumulator, so no temporary memory is needed.
@@ -775,12 +1018,18 @@ int build_bounds_only(void *d_bounds, const void *d_points, int n, const cukd_la
                       hipStream_t s)
 {
   uint32_t *acc = (uint32_t *)d_bounds;
-  hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, acc, L.num_dims);
-  int blocks = (n + 255) / 256;
-  if (blocks > 2048) blocks = 2048;
-  hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, s, acc, (const uint8_t *)d_points, n,
-                     L);
-  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, s, (const uint32_t *)acc, acc,
+  TreeSet<InTree> one;
+  one.tab = nullptr;
+  one.count = 1;
+  one.one.n = n;
+  one.one.base = 0;
+  one.one.id = 0;
+  one.one.seg = 0;
+  one.one.tile0 = 0;
+  hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, acc, L.num_dims, 1);
+  hipLaunchKernelGGL(k_bounds, dim3((n + IN_TILE - 1) / IN_TILE), dim3(256), 0, s, one, acc,
+                     (const uint8_t *)d_points, L);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, s, one, 1, (const uint32_t *)acc, acc,
                      (SegState *)nullptr, L, 0);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -792,97 +1041,23 @@ int build_bounds_only(void *d_bounds, const void *d_points, int n, const cukd_la
 
 size_t build_temp_bytes(int n, const cukd_layout &L)
 {
-  BuildPlan P;
-  make_plan(n, L, P);
+  ForestPlan P;
+  const long long off[2] = {0, n};
+  make_forest_plan(P, off, 1, L, false);
   return P.total;
 }
 
-int build_tree_impl(void *d_points, int n, const cukd_layout &L,
-                    void *d_bounds, hipStream_t s, const MemResource &mem)
+int build_tree_impl(void *d_points, int n, const cukd_layout &L, void *d_bounds, hipStream_t s,
+                    const MemResource &mem)
 {
-  const bool expl = L.dim_offset >= 0;
-  const bool isf = L.scalar_type == CUKD_SCALAR_F32;
-  (void)isf;
-  BuildPlan P;
-  make_plan(n, L, P);
-  uint8_t *tmp = nullptr;
-  if (mem.alloc((void **)&tmp, P.total, s) != 0) return CUKD_ERR_ALLOC;
-  uint8_t *dataA = tmp + P.off_dataA, *dataB = tmp + P.off_dataB;
-  uint32_t *idxA = (uint32_t *)(tmp + P.off_idxA), *idxB = (uint32_t *)(tmp + P.off_idxB);
-  uint32_t *keyA = (uint32_t *)(tmp + P.off_keyA), *keyB = (uint32_t *)(tmp + P.off_keyB);
-  SegState *st[2] = {(SegState *)(tmp + P.off_st0), (SegState *)(tmp + P.off_st1)};
-  uint32_t *hist = (uint32_t *)(tmp + P.off_hist);
-  uint32_t *cursors = (uint32_t *)(tmp + P.off_cursors);
-  uint32_t *need = (uint32_t *)(tmp + P.off_need);
-  uint32_t *acc = (uint32_t *)(tmp + P.off_acc);
-  uint8_t *pts = (uint8_t *)d_points;
-  const int D = L.num_dims;
-
-  // zero the small state area (states, histograms, cursors, flags)
-  {
-    const size_t words = (P.off_acc - P.off_st0) / 4;
-    hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, s, (uint32_t *)(tmp + P.off_st0), words);
-  }
-  if (d_bounds || expl) {
-    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, acc, D);
-    int blocks = (n + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
-    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, s, acc, (const uint8_t *)pts, n, L);
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, s, (const uint32_t *)acc,
-                       (uint32_t *)d_bounds, st[0], L, expl ? 1 : 0);
-  }
-  hipLaunchKernelGGL(k_copy_in, dim3((n + 255) / 256), dim3(256), 0, s, dataA, idxA,
-                     P.L0 > 0 ? keyA : (uint32_t *)nullptr, (const uint8_t *)pts, n, L,
-                     (const SegState *)st[0], expl ? 1 : 0);
-  uint8_t *dcur = dataA, *dnext = dataB;
-  uint32_t *icur = idxA, *inext = idxB, *kcur = keyA, *knext = keyB;
-  for (int level = 0; level < P.L0; level++) {
-    LevelGeom g, gn;
-    g.init(n, level);
-    gn.init(n, level + 1);
-    SegState *sc = st[level & 1], *sn = st[(level + 1) & 1];
-    hipLaunchKernelGGL(k_level_init, dim3(((1 << level) + 255) / 256), dim3(256), 0, s,
-                       sc, cursors, need, n, level, D, expl ? 1 : 0);
-    const int tiles = (g.n_elems + P.T - 1) / P.T;
-    // select tiles: larger than the partition's (fewer, fatter histogram
-    // flushes), still at most two subtrees per tile (T <= F/2 <= min size)
-    int Tsel = 16384;
-    while (Tsel > g.F / 2) Tsel >>= 1;
-    const int stiles = (g.n_elems + Tsel - 1) / Tsel;
-    const int max_words = expl ? D + 1 : (level + 1 < D ? level + 1 : D) + 1;
-    for (int w = 0; w < max_words; w++) {
-      const int shifts[3] = {21, 10, 0};
-      const int bitsv[3] = {11, 11, 10};
-      for (int d = 0; d < 3; d++)
-        hipLaunchKernelGGL(k_select_pass, dim3(stiles), dim3(SEL_NT), 0, s, g, w, shifts[d],
-                           bitsv[d], Tsel, (const uint32_t *)kcur, (const uint8_t *)dcur,
-                           (const uint32_t *)icur, sc, sn, hist, need, L, expl ? 1 : 0);
-    }
-    hipLaunchKernelGGL(k_partition, dim3(tiles), dim3(SEL_NT), 0, s, g, gn, P.T,
-                       (const uint32_t *)kcur, (const uint8_t *)dcur, (const uint32_t *)icur,
-                       (const SegState *)sc, (const SegState *)sn, dnext, inext, knext, cursors,
-                       pts, L, expl ? 1 : 0, level + 1 < P.L0 ? 1 : 0);
-    uint8_t *t8 = dcur; dcur = dnext; dnext = t8;
-    uint32_t *t32 = icur; icur = inext; inext = t32;
-    t32 = kcur; kcur = knext; knext = t32;
-  }
-  const int blocks = 1 << P.L0;
-  const SegState *sfin = st[P.L0 & 1];
-  hipError_t e = hipSuccess;
-  switch (P.cap) {
-    case 512: e = launch_finish<512, 256>(expl, blocks, s, n, P.L0, dcur, icur, sfin, pts, L); break;
-    case 1024: e = launch_finish<1024, 256>(expl, blocks, s, n, P.L0, dcur, icur, sfin, pts, L); break;
-    case 2048: e = launch_finish<2048, 512>(expl, blocks, s, n, P.L0, dcur, icur, sfin, pts, L); break;
-    case 4096: e = launch_finish<4096, 512>(expl, blocks, s, n, P.L0, dcur, icur, sfin, pts, L); break;
-    default: e = launch_finish<8192, 512>(expl, blocks, s, n, P.L0, dcur, icur, sfin, pts, L); break;
-  }
-  if (e == hipSuccess) e = hipGetLastError();
-  mem.release(tmp, s);
-  if (e != hipSuccess) {
-    set_last_error("builder kernel launch failed: %s", hipGetErrorString(e));
-    return CUKD_ERR_HIP;
-  }
-  return CUKD_OK;
+  ForestPlan P;
+  const long long off[2] = {0, n};
+  make_forest_plan(P, off, 1, L, false);
+  uint8_t *ws = nullptr;
+  if (mem.alloc((void **)&ws, P.total, s) != 0) return CUKD_ERR_ALLOC;
+  const int rc = forest_build_run(P, d_points, d_bounds, ws, s);
+  mem.release(ws, s);
+  return rc;
 }
 
 }  // namespace cukd_amd

@@ -7,6 +7,7 @@
 
 #include "cukd_common.h"
 #include "query_impl.h"
+#include "forest_plan.h"
 
 namespace cukd_amd {
 
@@ -50,6 +51,9 @@ void MemResource::release(void *p, hipStream_t s) const
   (void)hipFreeAsync(p, s);
 }
 
+int build_tree_tl(void *d_points, int n, const cukd_layout &L, void *d_bounds, hipStream_t s,
+                  const MemResource &mem, bool carry_rank);
+size_t build_tl_temp_bytes(int n, const cukd_layout &L, bool carry_rank);
 int build_tree_impl(void *d_points, int n, const cukd_layout &L, void *d_bounds,
                     hipStream_t s, const MemResource &mem);
 size_t build_temp_bytes(int n, const cukd_layout &L);
@@ -127,6 +131,16 @@ static int check_layout(const cukd_layout *L)
 
 using namespace cukd_amd;
 
+// opaque forest handle of the C ABI
+struct cukd_forest {
+  ForestPlan plan;
+  uint8_t *ws = nullptr;        // build workspace, plan tables, tree offsets
+  int32_t *tree_off = nullptr;  // num_trees + 1 element offsets (device)
+  size_t bytes = 0;
+  cukd_mem_resource mem{};
+  bool has_mem = false;
+};
+
 extern "C" {
 
 const char *cukd_last_error(void) { return g_last_error; }
@@ -145,8 +159,14 @@ int cukd_build_tree_temp_bytes(int32_t n, const cukd_layout *layout, int builder
   g_last_error[0] = 0;
   if (int rc = check_layout(layout)) return rc;
   if (!bytes) { set_last_error("bytes is null"); return CUKD_ERR_INVALID; }
-  (void)builder;
-  *bytes = n < 1 ? 0 : build_temp_bytes(n, *layout);
+  if (builder < CUKD_BUILDER_DEFAULT || builder > CUKD_BUILDER_INPLACE) {
+    set_last_error("unknown builder %d", builder);
+    return CUKD_ERR_INVALID;
+  }
+  if (n < 1) *bytes = 0;
+  else if (builder == CUKD_BUILDER_BITONIC) *bytes = build_tl_temp_bytes(n, *layout, true);
+  else if (builder == CUKD_BUILDER_INPLACE) *bytes = 0;
+  else *bytes = build_temp_bytes(n, *layout);
   return CUKD_OK;
 }
 
@@ -171,6 +191,9 @@ int cukd_build_tree(void *d_points, int32_t n, const cukd_layout *layout, int bu
     return CUKD_ERR_NO_BOUNDS;
   }
   MemResource mr{mem};
+  if (builder == CUKD_BUILDER_BITONIC || builder == CUKD_BUILDER_INPLACE)
+    return build_tree_tl(d_points, n, *layout, d_world_bounds, stream, mr,
+                         builder == CUKD_BUILDER_BITONIC);
   return build_tree_impl(d_points, n, *layout, d_world_bounds, stream, mr);
 }
 
@@ -257,6 +280,8 @@ int cukd_fcp(int traversal, const void *d_tree, int32_t n, const cukd_layout *la
   a.ids = d_ids;
   a.dist2 = d_dist2;
   a.ret = nullptr;
+  a.la = ListArgs{nullptr, 0, 0};
+  a.fq = ForestQ{nullptr, nullptr, 0, nullptr};
   a.stats = g_stats;
   const hipError_t e = launch_query(layout->num_dims, traversal, a, stream);
   if (otmp) mr.release(otmp, stream);
@@ -300,11 +325,70 @@ int cukd_knn(int traversal, int list_kind, int k, const void *d_tree, int32_t n,
   a.ids = d_ids;
   a.dist2 = d_dist2;
   a.ret = d_max_radius2;
+  a.la = ListArgs{nullptr, 0, 0};
+  a.fq = ForestQ{nullptr, nullptr, 0, nullptr};
   a.stats = g_stats;
   const hipError_t e = launch_query(layout->num_dims, traversal, a, stream);
   if (otmp) mr.release(otmp, stream);
   if (e != hipSuccess) {
     set_last_error("knn launch failed: %s", hipGetErrorString(e));
+    return CUKD_ERR_HIP;
+  }
+  return CUKD_OK;
+}
+
+int cukd_knn_accumulate(int traversal, int k, const void *d_tree, int32_t n,
+                        const cukd_layout *layout, const void *d_world_bounds,
+                        const float *d_queries, int32_t nq, uint64_t *d_lists, int init,
+                        float cutoff, int32_t id_offset, int flags, hipStream_t stream,
+                        const cukd_mem_resource *mem)
+{
+  g_last_error[0] = 0;
+  if (nq > 0 && !d_lists) { set_last_error("d_lists is null"); return CUKD_ERR_INVALID; }
+  if (int rc = check_query_common(traversal, d_tree, n, layout, d_world_bounds, d_queries,
+                                  nq, (int32_t *)d_lists))
+    return rc;
+  if (k < 1 || k > 64) { set_last_error("k = %d not in 1..64", k); return CUKD_ERR_INVALID; }
+  if (nq == 0) return CUKD_OK;
+  MemResource mr{mem};
+  void *otmp = nullptr;
+  if ((flags & CUKD_QUERY_SORT) && mr.alloc(&otmp, query_order_temp_bytes(nq), stream) != 0)
+    return CUKD_ERR_ALLOC;
+  QueryArgs a;
+  a.t = make_view(d_tree, n, layout);
+  a.queries = d_queries;
+  a.nq = nq;
+  a.world_bounds = (const float *)d_world_bounds;
+  a.cutoff = cutoff;
+  a.eps = 0.f;
+  a.k = k;
+  a.list_kind = LIST_ACCUM;
+  a.order = otmp ? make_query_order(otmp, d_queries, nq, layout->num_dims, stream) : nullptr;
+  a.ids = nullptr;
+  a.dist2 = nullptr;
+  a.ret = nullptr;
+  a.la = ListArgs{d_lists, init ? 1 : 0, id_offset};
+  a.fq = ForestQ{nullptr, nullptr, 0, nullptr};
+  a.stats = g_stats;
+  const hipError_t e = launch_query(layout->num_dims, traversal, a, stream);
+  if (otmp) mr.release(otmp, stream);
+  if (e != hipSuccess) {
+    set_last_error("knn accumulate launch failed: %s", hipGetErrorString(e));
+    return CUKD_ERR_HIP;
+  }
+  return CUKD_OK;
+}
+
+int cukd_knn_extract(const uint64_t *d_lists, int32_t nq, int k, int32_t *d_ids,
+                     float *d_dist2, float *d_radius, hipStream_t stream)
+{
+  g_last_error[0] = 0;
+  if (k < 1 || k > 64) { set_last_error("k = %d not in 1..64", k); return CUKD_ERR_INVALID; }
+  if (nq < 0 || (nq > 0 && !d_lists)) { set_last_error("bad lists"); return CUKD_ERR_INVALID; }
+  if (nq == 0) return CUKD_OK;
+  const hipError_t e = launch_knn_extract(d_lists, nq, k, d_ids, d_dist2, d_radius, stream);
+  if (e != hipSuccess) {
+    set_last_error("knn extract launch failed: %s", hipGetErrorString(e));
     return CUKD_ERR_HIP;
   }
   return CUKD_OK;
@@ -405,6 +489,175 @@ int cukd_spatial_knn(int traversal, int list_kind, int k, const cukd_spatial_tre
   if (nq > 0 && !d_dist2) { set_last_error("d_dist2 is null"); return CUKD_ERR_INVALID; }
   return spatial_query(traversal, list_kind, k, tree, layout, d_queries, nq, cutoff, d_ids,
                        d_dist2, d_max_radius2, stream);
+}
+
+// ---------------------------------------------------------------------------
+// forests: batched build and queries over many trees
+// ---------------------------------------------------------------------------
+
+int cukd_forest_create(cukd_forest **forest, const int64_t *h_offsets, int32_t num_trees,
+                       const cukd_layout *layout, hipStream_t stream,
+                       const cukd_mem_resource *mem)
+{
+  g_last_error[0] = 0;
+  if (!forest) { set_last_error("forest is null"); return CUKD_ERR_INVALID; }
+  *forest = nullptr;
+  if (int rc = check_layout(layout)) return rc;
+  if (num_trees < 1 || !h_offsets) {
+    set_last_error("need num_trees >= 1 and an offsets array");
+    return CUKD_ERR_INVALID;
+  }
+  if (h_offsets[0] != 0) { set_last_error("offsets must start at 0"); return CUKD_ERR_INVALID; }
+  for (int t = 0; t < num_trees; t++) {
+    const int64_t n = h_offsets[t + 1] - h_offsets[t];
+    if (n < 0 || n >= (1 << 30)) {
+      set_last_error("tree %d has %lld points (must be 0 .. 2^30-1)", t, (long long)n);
+      return CUKD_ERR_INVALID;
+    }
+  }
+  if (h_offsets[num_trees] >= ((int64_t)1 << 31)) {
+    set_last_error("forest of %lld points exceeds 2^31-1", (long long)h_offsets[num_trees]);
+    return CUKD_ERR_INVALID;
+  }
+  cukd_forest *f = new cukd_forest();
+  f->has_mem = mem != nullptr;
+  if (mem) f->mem = *mem;
+  std::vector<long long> off(num_trees + 1);
+  for (int t = 0; t <= num_trees; t++) off[t] = (long long)h_offsets[t];
+  make_forest_plan(f->plan, off.data(), num_trees, *layout, true);
+  const size_t off_trees = (f->plan.total + 255) & ~(size_t)255;
+  f->bytes = off_trees + (size_t)(num_trees + 1) * 4;
+  MemResource mr{f->has_mem ? &f->mem : nullptr};
+  if (mr.alloc((void **)&f->ws, f->bytes, stream) != 0) {
+    delete f;
+    return CUKD_ERR_ALLOC;
+  }
+  f->tree_off = (int32_t *)(f->ws + off_trees);
+  std::vector<uint8_t> img;
+  size_t begin = 0;
+  forest_tables_image(f->plan, img, begin);
+  std::vector<int32_t> toff(num_trees + 1);
+  for (int t = 0; t <= num_trees; t++) toff[t] = (int32_t)h_offsets[t];
+  hipError_t e = hipSuccess;
+  if (!img.empty()) e = hipMemcpyAsync(f->ws + begin, img.data(), img.size(), hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(f->tree_off, toff.data(), toff.size() * 4, hipMemcpyHostToDevice, stream);
+  // the host images go out of scope here
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    set_last_error("forest set-up failed: %s", hipGetErrorString(e));
+    mr.release(f->ws, stream);
+    delete f;
+    return CUKD_ERR_HIP;
+  }
+  *forest = f;
+  return CUKD_OK;
+}
+
+int cukd_forest_workspace_bytes(const cukd_forest *forest, size_t *bytes)
+{
+  g_last_error[0] = 0;
+  if (!forest || !bytes) { set_last_error("null argument"); return CUKD_ERR_INVALID; }
+  *bytes = forest->bytes;
+  return CUKD_OK;
+}
+
+int cukd_forest_build(cukd_forest *forest, void *d_points, void *d_world_bounds,
+                      hipStream_t stream)
+{
+  g_last_error[0] = 0;
+  if (!forest) { set_last_error("forest is null"); return CUKD_ERR_INVALID; }
+  if (forest->plan.ntotal > 0 && !d_points) {
+    set_last_error("d_points is null");
+    return CUKD_ERR_INVALID;
+  }
+  if (forest->plan.expl && !d_world_bounds) {
+    set_last_error("asked to build k-d trees over nodes with explicit dims, "
+                   "but no memory for world bounds provided");
+    return CUKD_ERR_NO_BOUNDS;
+  }
+  return forest_build_run(forest->plan, d_points, d_world_bounds, forest->ws, stream);
+}
+
+int cukd_forest_destroy(cukd_forest *forest, hipStream_t stream)
+{
+  g_last_error[0] = 0;
+  if (!forest) return CUKD_OK;
+  MemResource mr{forest->has_mem ? &forest->mem : nullptr};
+  mr.release(forest->ws, stream);
+  delete forest;
+  return CUKD_OK;
+}
+
+static int forest_query(int traversal, int list_kind, int k, const cukd_forest *forest,
+                        const void *d_points, const void *d_world_bounds,
+                        const float *d_queries, int32_t nq, int32_t num_segments,
+                        const int32_t *d_seg_offsets, const int32_t *d_seg_tree, float cutoff,
+                        float eps, int32_t *d_ids, float *d_dist2, float *d_ret,
+                        hipStream_t stream)
+{
+  if (!forest) { set_last_error("forest is null"); return CUKD_ERR_INVALID; }
+  const cukd_layout &L = forest->plan.L;
+  if (int rc = check_query_common(traversal, d_points, (int32_t)(forest->plan.ntotal > 0), &L,
+                                  d_world_bounds, d_queries, nq, d_ids))
+    return rc;
+  if (nq == 0) return CUKD_OK;
+  if (num_segments < 1 || !d_seg_offsets || !d_seg_tree) {
+    set_last_error("need num_segments >= 1 with segment offsets and trees");
+    return CUKD_ERR_INVALID;
+  }
+  QueryArgs a;
+  a.t = make_view(d_points, 0, &L);
+  a.queries = d_queries;
+  a.nq = nq;
+  a.world_bounds = (const float *)d_world_bounds;
+  a.cutoff = cutoff;
+  a.eps = eps;
+  a.k = k;
+  a.list_kind = list_kind;
+  a.order = nullptr;
+  a.ids = d_ids;
+  a.dist2 = d_dist2;
+  a.ret = d_ret;
+  a.la = ListArgs{nullptr, 0, 0};
+  a.fq = ForestQ{d_seg_offsets, d_seg_tree, num_segments, forest->tree_off};
+  a.stats = g_stats;
+  const hipError_t e = launch_query(L.num_dims, traversal, a, stream);
+  if (e != hipSuccess) {
+    set_last_error("forest query launch failed: %s", hipGetErrorString(e));
+    return CUKD_ERR_HIP;
+  }
+  return CUKD_OK;
+}
+
+int cukd_forest_fcp(int traversal, const cukd_forest *forest, const void *d_points,
+                    const void *d_world_bounds, const float *d_queries, int32_t nq,
+                    int32_t num_segments, const int32_t *d_seg_offsets,
+                    const int32_t *d_seg_tree, float cutoff, float eps, int32_t *d_ids,
+                    float *d_dist2, hipStream_t stream)
+{
+  g_last_error[0] = 0;
+  return forest_query(traversal, -1, 1, forest, d_points, d_world_bounds, d_queries, nq,
+                      num_segments, d_seg_offsets, d_seg_tree, cutoff, eps, d_ids, d_dist2,
+                      nullptr, stream);
+}
+
+int cukd_forest_knn(int traversal, int list_kind, int k, const cukd_forest *forest,
+                    const void *d_points, const void *d_world_bounds, const float *d_queries,
+                    int32_t nq, int32_t num_segments, const int32_t *d_seg_offsets,
+                    const int32_t *d_seg_tree, float cutoff, int32_t *d_ids, float *d_dist2,
+                    float *d_max_radius2, hipStream_t stream)
+{
+  g_last_error[0] = 0;
+  if (k < 1 || k > 64) { set_last_error("k = %d not in 1..64", k); return CUKD_ERR_INVALID; }
+  if (list_kind != CUKD_LIST_FIXED && list_kind != CUKD_LIST_HEAP) {
+    set_last_error("unknown list kind %d", list_kind);
+    return CUKD_ERR_INVALID;
+  }
+  if (nq > 0 && !d_dist2) { set_last_error("d_dist2 is null"); return CUKD_ERR_INVALID; }
+  return forest_query(traversal, list_kind, k, forest, d_points, d_world_bounds, d_queries, nq,
+                      num_segments, d_seg_offsets, d_seg_tree, cutoff, 0.f, d_ids, d_dist2,
+                      d_max_radius2, stream);
 }
 
 }  // extern "C"

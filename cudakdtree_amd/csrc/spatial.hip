@@ -498,12 +498,13 @@ __global__ __launch_bounds__(128) void k_spatial_query(const cukd_spatial_node *
 #pragma unroll
   for (int d = 0; d < D; d++) q[d] = queries[(size_t)i * D + d];
   typename ResultOps<R>::State st;
-  R &res = ResultOps<R>::init(st, cutoff, k);
+  const ListArgs no_lists{nullptr, 0, 0};
+  R &res = ResultOps<R>::init(st, cutoff, k, i, no_lists);
   float lo[D], hi[D];
 #pragma unroll
   for (int d = 0; d < D; d++) { lo[d] = bounds.v[d]; hi[d] = bounds.v[D + d]; }
   const bool ok = cukd::detail::spatial_traverse<D, CCT>(res, q, nodes, prim_ids, lo, hi, data);
-  ResultOps<R>::write(st, i, k, ids, dist2, ret);
+  ResultOps<R>::write(st, i, k, ids, dist2, ret, no_lists);
   // stack overflow: fcp reports no result, knn returns -1 (fcp.h:317-320,
   // knn.h:620-623 print "STACK OVERFLOW" and return -1)
   if (!ok) {

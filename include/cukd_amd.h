@@ -88,9 +88,20 @@ typedef struct cukd_mem_resource {
  * builder_inplace.h:82-102).  d_world_bounds (box of 2*num_dims scalars,
  * lower then upper) receives the bounds when non-null and is required for
  * explicit-dim layouts (builder_thrust.h:469-473).  n < 1 returns CUKD_OK
- * without work (builder_thrust.h:441).  The resulting array is identical
- * to buildTree_host's (builder_thrust.h:514-601) for every input without
- * NaN coordinates, including inputs with tied coordinates. */
+ * without work (builder_thrust.h:441).
+ *
+ * CUKD_BUILDER_DEFAULT / _THRUST: radix-select builder, about 2.3n words
+ *   of temporary memory; the array is identical to buildTree_host's
+ *   (builder_thrust.h:514-601) for every input without NaN coordinates,
+ *   ties included.
+ * CUKD_BUILDER_BITONIC: sorts each subtree inside the final array with a
+ *   bitonic network; temporary memory n ints (builder.h:34-38), identical
+ *   to buildTree_host's like the default builder.
+ * CUKD_BUILDER_INPLACE: same network, no temporary memory at all
+ *   (builder.h:41-45); ties on a split coordinate are broken by the
+ *   element's bytes, so the array equals buildTree_host's whenever the
+ *   split coordinates decide (always on tie-free inputs) and is otherwise
+ *   a valid, deterministic left-balanced tree. */
 int cukd_build_tree(void *d_points, int32_t n, const cukd_layout *layout,
                     int builder, void *d_world_bounds, hipStream_t stream,
                     const cukd_mem_resource *mem);
@@ -128,6 +139,28 @@ int cukd_knn(int traversal, int list_kind, int k, const void *d_tree,
              float cutoff, int32_t *d_ids, float *d_dist2,
              float *d_max_radius2, int flags, hipStream_t stream,
              const cukd_mem_resource *mem);
+
+/* One round of a multi-round kNN: every query continues its own
+ * FlexHeapCandidateList (knn.h:266-287, 360-421) held in HBM at
+ * d_lists[q*k .. q*k+k) (uint64 entries, dist2 bits << 32 | id), as the
+ * reference's runQuery kernel does per ring round
+ * (samples/mpiHugeQuery.cu:75-88).  init != 0 first fills the list with
+ * (cutoff^2, -1) (the reference's round 0); otherwise the list is used as
+ * it is.  Candidate ids are id_offset + tree index (the reference's rounds
+ * use 0, leaving ids local to each tree).  k in 1..64. */
+int cukd_knn_accumulate(int traversal, int k, const void *d_tree, int32_t n,
+                        const cukd_layout *layout, const void *d_world_bounds,
+                        const float *d_queries, int32_t nq, uint64_t *d_lists,
+                        int init, float cutoff, int32_t id_offset, int flags,
+                        hipStream_t stream, const cukd_mem_resource *mem);
+
+/* extractFinalResult (samples/mpiHugeQuery.cu:90-103): d_radius[q] =
+ * sqrt of the list's max dist2 (infinity stays infinity).  d_ids/d_dist2
+ * (each nullable, nq*k) receive the entries in ascending (dist2, id)
+ * order. */
+int cukd_knn_extract(const uint64_t *d_lists, int32_t nq, int k,
+                     int32_t *d_ids, float *d_dist2, float *d_radius,
+                     hipStream_t stream);
 
 /* Spatial k-d tree (spatial-kdtree.h:69-98).  Node = {float pos;
  * uint32 offset; uint16 count; int16 dim} (12 bytes). */
@@ -171,6 +204,55 @@ int cukd_spatial_knn(int traversal, int list_kind, int k,
                      const float *d_queries, int32_t nq, float cutoff,
                      int32_t *d_ids, float *d_dist2, float *d_max_radius2,
                      hipStream_t stream);
+
+/* ---- forests: many independent trees, built and queried together ----
+ *
+ * No counterpart in the reference, which builds and queries one tree per
+ * call; every tree of a forest is the tree cukd_build_tree builds over the
+ * same points, and every query result equals cukd_fcp / cukd_knn on that
+ * tree alone (ids index the query's own tree). */
+typedef struct cukd_forest cukd_forest;
+
+/* Plans a forest: tree t is d_points[h_offsets[t] .. h_offsets[t+1]) of
+ * one array (h_offsets: host, num_trees+1 values, starting at 0,
+ * non-decreasing, every tree < 2^30 points, all < 2^31).  Allocates the
+ * whole build workspace once through mem (NULL: hipMallocAsync) and
+ * synchronises stream once, so that cukd_forest_build allocates nothing,
+ * never synchronises, and can be captured into a HIP graph. */
+int cukd_forest_create(cukd_forest **forest, const int64_t *h_offsets,
+                       int32_t num_trees, const cukd_layout *layout,
+                       hipStream_t stream, const cukd_mem_resource *mem);
+
+/* Device bytes held by the forest's workspace. */
+int cukd_forest_workspace_bytes(const cukd_forest *forest, size_t *bytes);
+
+/* Builds every tree of the forest in place, in one sequence of launches
+ * for all trees.  d_world_bounds (nullable unless explicit dims) receives
+ * one box per tree, num_trees * 2 * num_dims scalars. */
+int cukd_forest_build(cukd_forest *forest, void *d_points,
+                      void *d_world_bounds, hipStream_t stream);
+
+int cukd_forest_destroy(cukd_forest *forest, hipStream_t stream);
+
+/* Batched queries over a built forest in one launch.  The nq queries form
+ * num_segments segments: segment s is queries
+ * d_seg_offsets[s] .. d_seg_offsets[s+1]) and runs against tree
+ * d_seg_tree[s] (both device arrays; a tree may appear in several
+ * segments).  d_world_bounds (per-tree boxes) is required for CUKD_CCT.
+ * Output layout and meaning as cukd_fcp / cukd_knn. */
+int cukd_forest_fcp(int traversal, const cukd_forest *forest,
+                    const void *d_points, const void *d_world_bounds,
+                    const float *d_queries, int32_t nq, int32_t num_segments,
+                    const int32_t *d_seg_offsets, const int32_t *d_seg_tree,
+                    float cutoff, float eps, int32_t *d_ids, float *d_dist2,
+                    hipStream_t stream);
+int cukd_forest_knn(int traversal, int list_kind, int k,
+                    const cukd_forest *forest, const void *d_points,
+                    const void *d_world_bounds, const float *d_queries,
+                    int32_t nq, int32_t num_segments,
+                    const int32_t *d_seg_offsets, const int32_t *d_seg_tree,
+                    float cutoff, int32_t *d_ids, float *d_dist2,
+                    float *d_max_radius2, hipStream_t stream);
 
 /* Traversal statistics, replacing CUKD_STATS / cukd::g_traversalStats
  * (common.h:348-360): while d_counter is non-null, every query launch adds

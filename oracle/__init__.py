@@ -57,6 +57,11 @@ def lib():
         L.oc_fcp.restype = None
         L.oc_knn.argtypes = [i32, i32, i32, P, i64, P, P, P, i64, f32, P, P, P, P, i32]
         L.oc_knn.restype = None
+        L.oc_knn_accumulate.argtypes = [i32, i32, P, i64, P, P, P, i64, P, i32, f32, i32, P,
+                                        i32]
+        L.oc_knn_accumulate.restype = None
+        L.oc_knn_extract.argtypes = [P, i64, i32, P, P, P]
+        L.oc_knn_extract.restype = None
         L.oc_spatial_build.argtypes = [P, i64, P, i32, P, P, P]
         L.oc_spatial_build.restype = i64
         L.oc_spatial_fcp.argtypes = [i32, P, P, P, P, P, P, i64, f32, P, P, i32]
@@ -161,6 +166,32 @@ def knn(method, list_kind, k, tree, layout, queries, cutoff=float("inf"),
                  _ptr(wb), _ptr(q), nq, cutoff, _ptr(ids), _ptr(d2), _ptr(ret),
                  _ptr(vis), _nthreads(threads))
     return (ids, d2, ret, vis) if want_visits else (ids, d2, ret)
+
+
+def knn_accumulate(method, k, tree, layout, queries, lists, init, cutoff=float("inf"),
+                   id_offset=0, world_bounds=None, threads=0):
+    """One round of kNN into persistent FlexHeap lists (uint64 (nq, k),
+    updated in place); see oc_knn_accumulate."""
+    tree = np.ascontiguousarray(tree)
+    q = np.ascontiguousarray(queries, dtype=np.float32)
+    assert lists.dtype == np.uint64 and lists.flags.c_contiguous and lists.shape == (len(q), k)
+    wb = None if world_bounds is None else np.ascontiguousarray(world_bounds, np.float32)
+    lay = _layout(layout)
+    lib().oc_knn_accumulate(method, k, _ptr(tree), len(tree), ctypes.byref(lay), _ptr(wb),
+                            _ptr(q), len(q), _ptr(lists), 1 if init else 0, cutoff, id_offset,
+                            None, _nthreads(threads))
+    return lists
+
+
+def knn_extract(lists):
+    """(ids, dist2) ascending per query and the extractFinalResult radius."""
+    lists = np.ascontiguousarray(lists, dtype=np.uint64)
+    nq, k = lists.shape
+    ids = np.empty((nq, k), np.int32)
+    d2 = np.empty((nq, k), np.float32)
+    rad = np.empty(nq, np.float32)
+    lib().oc_knn_extract(_ptr(lists), nq, k, _ptr(ids), _ptr(d2), _ptr(rad))
+    return ids, d2, rad
 
 
 def spatial_build(points, layout, leaf_threshold=0):

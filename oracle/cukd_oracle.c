@@ -353,6 +353,7 @@ int oc_build_host(void *points, int64_t n, const oc_layout *L,
 typedef struct {
   int kind;          /* 0 = FCP, 1 = fixed list, 2 = heap list */
   int k;
+  int id_offset;     /* added to every candidate id (multi-round lists) */
   uint64_t *entry;   /* k entries for kNN */
   int closest_id;    /* FCP */
   float closest_d2;  /* FCP */
@@ -387,7 +388,7 @@ static inline float result_process(result_t *r, int id, float d2)
     if (d2 < r->closest_d2) { r->closest_d2 = d2; r->closest_id = id; }
     return r->closest_d2;
   }
-  uint64_t v = enc(d2, id);
+  uint64_t v = enc(d2, id + r->id_offset);
   uint64_t *e = r->entry;
   const int k = r->k;
   if (r->kind == 1) {
@@ -606,7 +607,7 @@ void oc_fcp(int method, const void *tree, int64_t n, const oc_layout *L,
 #pragma omp parallel for schedule(dynamic, 256) num_threads(num_threads > 0 ? num_threads : 1)
   for (int64_t i = 0; i < nq; i++) {
     result_t r;
-    r.kind = 0; r.k = 1; r.entry = NULL;
+    r.kind = 0; r.k = 1; r.entry = NULL; r.id_offset = 0;
     result_init(&r, cutoff);
     const int64_t v = run_traversal(method, &r, queries + i * D, world_bounds,
                                     (const uint8_t *)tree, n, L, eps);
@@ -631,6 +632,7 @@ void oc_knn(int method, int list_kind, int k, const void *tree, int64_t n,
     result_t r;
     r.kind = list_kind == OC_LIST_HEAP ? 2 : 1;
     r.k = k;
+    r.id_offset = 0;
     r.entry = entries;
     result_init(&r, cutoff);
     const int64_t v = run_traversal(method, &r, queries + i * D, world_bounds,
@@ -641,6 +643,57 @@ void oc_knn(int method, int list_kind, int k, const void *tree, int64_t n,
     }
     if (ret) ret[i] = result_max_r2(&r);
     if (visits) visits[i] = v;
+  }
+}
+
+/* Multi-round kNN into persistent lists (samples/mpiHugeQuery.cu:75-103):
+ * each query's FlexHeapCandidateList (knn.h:266-287, 360-421) lives in
+ * lists[i*k .. i*k+k); init != 0 fills it with (cutoff^2, -1) first (the
+ * reference's round 0), otherwise the heap is continued as is.  Candidate
+ * ids are id_offset + tree index (the reference's rounds use offset 0). */
+void oc_knn_accumulate(int method, int k, const void *tree, int64_t n, const oc_layout *L,
+                       const void *world_bounds, const float *queries, int64_t nq,
+                       uint64_t *lists, int init, float cutoff, int32_t id_offset,
+                       int64_t *visits, int num_threads)
+{
+  const int D = L->num_dims;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(num_threads > 0 ? num_threads : 1)
+  for (int64_t i = 0; i < nq; i++) {
+    result_t r;
+    r.kind = 2;
+    r.k = k;
+    r.id_offset = id_offset;
+    r.entry = lists + i * k;
+    if (init) result_init(&r, cutoff);
+    const int64_t v = run_traversal(method, &r, queries + i * D, world_bounds,
+                                    (const uint8_t *)tree, n, L, 0.f);
+    if (visits) visits[i] = v;
+  }
+}
+
+/* extractFinalResult (samples/mpiHugeQuery.cu:90-103): radius = sqrt of
+ * the list's max dist2 unless infinite; ids/dist2 (nullable) receive the
+ * entries in ascending (dist2, id) order. */
+static int cmp_u64(const void *a, const void *b)
+{
+  const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+  return x < y ? -1 : x > y;
+}
+void oc_knn_extract(const uint64_t *lists, int64_t nq, int k, int32_t *ids, float *dist2,
+                    float *radius)
+{
+  uint64_t tmp[256];
+  for (int64_t i = 0; i < nq; i++) {
+    const uint64_t *e = lists + i * k;
+    float m = u2f((uint32_t)(e[0] >> 32));
+    if (radius) radius[i] = isinf(m) ? m : sqrtf(m);
+    if (!ids && !dist2) continue;
+    memcpy(tmp, e, (size_t)k * 8);
+    qsort(tmp, (size_t)k, 8, cmp_u64);
+    for (int j = 0; j < k; j++) {
+      if (ids) ids[i * k + j] = (int32_t)(uint32_t)tmp[j];
+      if (dist2) dist2[i * k + j] = u2f((uint32_t)(tmp[j] >> 32));
+    }
   }
 }
 
@@ -900,7 +953,7 @@ void oc_spatial_fcp(int method, const oc_spatial_node *nodes,
 #pragma omp parallel for schedule(dynamic, 256) num_threads(num_threads > 0 ? num_threads : 1)
   for (int64_t i = 0; i < nq; i++) {
     result_t r;
-    r.kind = 0; r.k = 1; r.entry = NULL;
+    r.kind = 0; r.k = 1; r.entry = NULL; r.id_offset = 0;
     result_init(&r, cutoff);
     const int st = spatial_traverse(method, &r, queries + i * D, nodes, prim_ids,
                                     bounds, (const uint8_t *)points, L);
@@ -923,6 +976,7 @@ void oc_spatial_knn(int method, int list_kind, int k,
     result_t r;
     r.kind = list_kind == OC_LIST_HEAP ? 2 : 1;
     r.k = k;
+    r.id_offset = 0;
     r.entry = entries;
     result_init(&r, cutoff);
     const int st = spatial_traverse(method, &r, queries + i * D, nodes, prim_ids,

@@ -65,6 +65,15 @@ void oc_knn(int method, int list_kind, int k, const void *tree, int64_t n,
             int32_t *ids, float *dist2, float *ret, int64_t *visits,
             int num_threads);
 
+/* multi-round kNN into persistent FlexHeap lists and their extraction
+ * (knn.h:266-287, 360-421; samples/mpiHugeQuery.cu:75-103) */
+void oc_knn_accumulate(int method, int k, const void *tree, int64_t n, const oc_layout *L,
+                       const void *world_bounds, const float *queries, int64_t nq,
+                       uint64_t *lists, int init, float cutoff, int32_t id_offset,
+                       int64_t *visits, int num_threads);
+void oc_knn_extract(const uint64_t *lists, int64_t nq, int k, int32_t *ids, float *dist2,
+                    float *radius);
+
 /* ---- spatial k-d tree (spatial-kdtree.h:69-105, 496-612) ---- */
 typedef struct oc_spatial_node {
   float    pos;

@@ -7,7 +7,9 @@ from cudakdtree_amd import datagen
 
 def layout_for(kind, d):
     """kind: 'float', 'int', 'payload', 'payload_int', 'explicit',
-    'explicit_int', 'photon' (float3 + uint8 dim at byte 12, stride 16)."""
+    'explicit_int', 'photon' (float3 + uint8 dim at byte 12, stride 16),
+    'rec96' (96-byte record, point at byte 32, no dim), 'rec256' (256-byte
+    record, point at byte 16, int32 dim at byte 200)."""
     if kind == "float":
         return ck.float_points(d)
     if kind == "int":
@@ -22,6 +24,10 @@ def layout_for(kind, d):
         return ck.point_plus_payload(d, ck.SCALAR_I32, explicit_dim=True)
     if kind == "photon":
         return ck.Layout(d, ck.SCALAR_F32, 4 * d + 4, 0, 4 * d, 1)
+    if kind == "rec96":
+        return ck.Layout(d, ck.SCALAR_F32, 96, 32, -1, 4)
+    if kind == "rec256":
+        return ck.Layout(d, ck.SCALAR_F32, 256, 16, 200, 4)
     raise ValueError(kind)
 
 
@@ -82,6 +88,17 @@ def distribution(name, n, d, seed, scalar_float=True):
     if not scalar_float:
         c = np.round(c * 1000).astype(np.int32)
     return c
+
+
+def same_multiset(a, b, layout):
+    """a and b hold the same elements, ignoring the dim field."""
+    def canon(x):
+        x = np.ascontiguousarray(x).copy()
+        if layout.dim_offset >= 0:
+            raw = x.view(np.uint8).reshape(len(x), layout.stride)
+            raw[:, layout.dim_offset:layout.dim_offset + layout.dim_bytes] = 0
+        return x[np.lexsort(x.T[::-1])]
+    return np.array_equal(canon(a), canon(b))
 
 
 def check_kdtree_valid(coords, dims, n):

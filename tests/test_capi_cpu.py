@@ -87,3 +87,16 @@ def test_temp_bytes():
     assert 0 < small < big
     # two ping-pong copies of (point, index, key) per element dominate
     assert big >= 2 * 1_000_000 * (12 + 8)
+
+
+def test_temp_bytes_per_builder():
+    """bitonic needs n ints once the tree has levels above the LDS finish;
+    in-place needs nothing (builder.h:34-45)."""
+    for lay in (ck.float_points(3), ck.point_plus_payload(4, explicit_dim=True)):
+        assert ck.build_temp_bytes(1_000_000, lay, ck.BUILDER_BITONIC) == 4_000_000
+        assert ck.build_temp_bytes(1_000_000, lay, ck.BUILDER_INPLACE) == 0
+        assert ck.build_temp_bytes(100, lay, ck.BUILDER_BITONIC) == 0
+        assert ck.build_temp_bytes(1_000_000, lay, ck.BUILDER_THRUST) == \
+            ck.build_temp_bytes(1_000_000, lay, ck.BUILDER_DEFAULT)
+    with pytest.raises(RuntimeError):
+        ck.build_temp_bytes(10, ck.float_points(3), 9)
