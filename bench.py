@@ -145,6 +145,9 @@ def main():
     ap.add_argument("--forest-trees", type=int, default=4096)
     ap.add_argument("--forest-steps", type=int, default=2)
     ap.add_argument("--no-forest", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="for hardware-counter runs: one 1 GiB device copy (the counters' "
+                         "calibration), one build, one kNN and one FCP launch, no JSON line")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -174,6 +177,22 @@ def main():
     tree = torch.empty_like(pts)
     torch.cuda.synchronize()
     print("[bench] inputs resident: %d points, %d queries" % (n, nq), file=sys.stderr, flush=True)
+
+    if args.probe:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        tree.copy_(pts)
+        ck.buildTree(tree, lay)
+        ids = torch.empty((nq, args.k), dtype=torch.int32, device=dev)
+        d2 = torch.empty((nq, args.k), dtype=torch.float32, device=dev)
+        ck.knn(tree, lay, queries, args.k, out=(ids, d2, None), flags=ck.QUERY_SORT)
+        ck.fcp(tree, lay, queries, flags=ck.QUERY_SORT)
+        torch.cuda.synchronize()
+        print("[bench] probe done", file=sys.stderr, flush=True)
+        return
 
     # ---------------- build: 10M float3 ----------------
     stream = torch.cuda.current_stream()

@@ -35,6 +35,7 @@ SCALAR_F32, SCALAR_I32 = 0, 1
 BUILDER_DEFAULT, BUILDER_THRUST, BUILDER_BITONIC, BUILDER_INPLACE = 0, 1, 2, 3
 STACK_BASED, STACK_FREE, CCT = 0, 1, 2
 LIST_FIXED, LIST_HEAP = 0, 1
+QUERY_AUTOTUNE = 2
 QUERY_SORT = 1
 
 # every entry point declared in include/cukd_amd.h
@@ -44,7 +45,9 @@ EXPORTED_SYMBOLS = (
     "cukd_spatial_fcp", "cukd_spatial_knn", "cukd_set_traversal_stats",
     "cukd_knn_accumulate", "cukd_knn_extract",
     "cukd_forest_create", "cukd_forest_workspace_bytes", "cukd_forest_build",
-    "cukd_forest_destroy", "cukd_forest_fcp", "cukd_forest_knn",
+    "cukd_forest_destroy", "cukd_forest_fcp", "cukd_forest_knn", "cukd_forest_reserve_queries",
+    "cukd_server_create", "cukd_server_submit", "cukd_server_poll",
+    "cukd_server_get_stats", "cukd_server_destroy",
     "cukd_last_error", "cukd_version",
 )
 
@@ -155,9 +158,19 @@ def lib():
     L.cukd_forest_workspace_bytes.argtypes = [P, ctypes.POINTER(ctypes.c_size_t)]
     L.cukd_forest_build.argtypes = [P, P, P, P]
     L.cukd_forest_destroy.argtypes = [P, P]
-    L.cukd_forest_fcp.argtypes = [ctypes.c_int, P, P, P, P, i32, i32, P, P, f32, f32, P, P, P]
+    L.cukd_forest_fcp.argtypes = [ctypes.c_int, P, P, P, P, i32, i32, P, P, f32, f32, P, P,
+                                  ctypes.c_int, P]
     L.cukd_forest_knn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, P, i32, i32,
-                                  P, P, f32, P, P, P, P]
+                                  P, P, f32, P, P, P, ctypes.c_int, P]
+    L.cukd_forest_reserve_queries.argtypes = [P, i32, P]
+    L.cukd_server_create.argtypes = [ctypes.POINTER(P), P, P, P, ctypes.c_int, ctypes.c_size_t,
+                                     i32, P]
+    L.cukd_server_submit.argtypes = [P, i32, P, i32, ctypes.c_int, f32, P, P,
+                                     ctypes.POINTER(ctypes.c_uint64)]
+    L.cukd_server_poll.argtypes = [P, ctypes.POINTER(Completion), i32,
+                                   ctypes.POINTER(ctypes.c_int32), i32]
+    L.cukd_server_get_stats.argtypes = [P, ctypes.POINTER(ServerStats)]
+    L.cukd_server_destroy.argtypes = [P]
     L.cukd_set_traversal_stats.argtypes = [P]
     L.cukd_last_error.restype = ctypes.c_char_p
     L.cukd_last_error.argtypes = []
@@ -320,6 +333,78 @@ def knn_extract(lists, stream=None):
     return ids, d2, rad
 
 
+class Completion(ctypes.Structure):
+    """cukd_completion"""
+    _fields_ = [("ticket", ctypes.c_uint64), ("status", ctypes.c_int32),
+                ("num_queries", ctypes.c_int32), ("queue_ms", ctypes.c_float),
+                ("total_ms", ctypes.c_float)]
+
+
+class ServerStats(ctypes.Structure):
+    """cukd_server_stats"""
+    _fields_ = [("requests_submitted", ctypes.c_uint64), ("requests_done", ctypes.c_uint64),
+                ("queries_done", ctypes.c_uint64), ("batches", ctypes.c_uint64)]
+
+
+class Server:
+    """Serving loop over a built forest (cukd_server_*).  Requests carry host
+    (numpy) queries; results land in numpy arrays owned by the request."""
+
+    def __init__(self, forest, points, world_bounds=None, traversal=STACK_BASED,
+                 budget_bytes=256 << 20, max_requests_per_batch=0, stream=None):
+        self.forest = forest
+        self._keep = {}
+        self._h = ctypes.c_void_p()
+        self._refs = (points, world_bounds)
+        _check(lib().cukd_server_create(ctypes.byref(self._h), forest._h, _ptr(points),
+                                        _ptr(world_bounds), traversal, budget_bytes,
+                                        max_requests_per_batch, _stream(stream)))
+
+    def submit(self, tree, queries, k=0, cutoff=math.inf):
+        """queries: float32 numpy (nq, D).  Returns the ticket; results are
+        in result(ticket) once polled complete."""
+        import numpy as np
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = len(q)
+        w = max(1, k)
+        ids = np.empty((nq, w) if k else nq, np.int32)
+        d2 = np.empty((nq, w) if k else nq, np.float32)
+        t = ctypes.c_uint64(0)
+        _check(lib().cukd_server_submit(self._h, tree, q.ctypes.data, nq, k, cutoff,
+                                        ids.ctypes.data, d2.ctypes.data, ctypes.byref(t)))
+        self._keep[t.value] = (q, ids, d2)
+        return t.value
+
+    def poll(self, wait_for=0, max_items=4096):
+        """List of completed Completion records (waits for wait_for)."""
+        buf = (Completion * max_items)()
+        cnt = ctypes.c_int32(0)
+        _check(lib().cukd_server_poll(self._h, buf, max_items, ctypes.byref(cnt), wait_for))
+        return [buf[i] for i in range(cnt.value)]
+
+    def result(self, ticket):
+        """(ids, dist2) of a completed request; releases its buffers."""
+        q, ids, d2 = self._keep.pop(ticket)
+        return ids, d2
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().cukd_server_get_stats(self._h, ctypes.byref(s)))
+        return s
+
+    def close(self):
+        if self._h:
+            rc = lib().cukd_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+            _check(rc)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Forest:
     """Many independent trees in one array, built and queried together
     (cukd_forest_*).  ``offsets``: num_trees+1 element offsets (host)."""
@@ -350,8 +435,11 @@ class Forest:
                                        _stream(stream)))
         return points
 
+    def reserve_queries(self, max_queries, stream=None):
+        _check(lib().cukd_forest_reserve_queries(self._h, max_queries, _stream(stream)))
+
     def fcp(self, points, queries, seg_offsets, seg_tree, traversal=STACK_BASED,
-            cutoff=math.inf, eps=0.0, world_bounds=None, stream=None, out=None):
+            cutoff=math.inf, eps=0.0, world_bounds=None, flags=0, stream=None, out=None):
         import torch
         nq = queries.shape[0]
         if out is None:
@@ -361,12 +449,13 @@ class Forest:
             ids, d2 = out
         _check(lib().cukd_forest_fcp(traversal, self._h, _ptr(points), _ptr(world_bounds),
                                      _ptr(queries), nq, len(seg_tree), _ptr(seg_offsets),
-                                     _ptr(seg_tree), cutoff, eps, _ptr(ids), _ptr(d2),
+                                     _ptr(seg_tree), cutoff, eps, _ptr(ids), _ptr(d2), flags,
                                      _stream(stream)))
         return ids, d2
 
     def knn(self, points, queries, k, seg_offsets, seg_tree, traversal=STACK_BASED,
-            list_kind=LIST_FIXED, cutoff=math.inf, world_bounds=None, stream=None, out=None):
+            list_kind=LIST_FIXED, cutoff=math.inf, world_bounds=None, flags=0, stream=None,
+            out=None):
         import torch
         nq = queries.shape[0]
         if out is None:
@@ -378,7 +467,7 @@ class Forest:
         _check(lib().cukd_forest_knn(traversal, list_kind, k, self._h, _ptr(points),
                                      _ptr(world_bounds), _ptr(queries), nq, len(seg_tree),
                                      _ptr(seg_offsets), _ptr(seg_tree), cutoff, _ptr(ids),
-                                     _ptr(d2), _ptr(ret), _stream(stream)))
+                                     _ptr(d2), _ptr(ret), flags, _stream(stream)))
         return ids, d2, ret
 
     def close(self, stream=None):

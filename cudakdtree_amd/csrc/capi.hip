@@ -5,6 +5,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <map>
+#include <mutex>
+
 #include "cukd_common.h"
 #include "query_impl.h"
 #include "forest_plan.h"
@@ -139,7 +142,15 @@ struct cukd_forest {
   size_t bytes = 0;
   cukd_mem_resource mem{};
   bool has_mem = false;
+  void *order_ws = nullptr;     // Morton-ordering scratch (cukd_forest_reserve_queries)
+  size_t order_ws_bytes = 0;
 };
+
+namespace cukd_amd {
+void clear_last_error() { g_last_error[0] = 0; }
+int forest_num_dims(const cukd_forest *f) { return f->plan.L.num_dims; }
+int forest_num_trees(const cukd_forest *f) { return f->plan.ntrees; }
+}  // namespace cukd_amd
 
 extern "C" {
 
@@ -213,6 +224,120 @@ int cukd_compute_bounds(void *d_bounds, const void *d_points, int32_t n,
   return build_bounds_only(d_bounds, d_points, n, *layout, stream);
 }
 
+// ---------------------------------------------------------------------------
+// query launches: Morton ordering, workgroup size and the autotuner
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct TuneKey {
+  int D, trav, list, k, bucket, forest;
+  bool operator<(const TuneKey &o) const
+  {
+    return memcmp(this, &o, sizeof(TuneKey)) < 0;
+  }
+};
+struct TuneVal {
+  int block, sort;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+
+int log2_bucket(int n)
+{
+  int b = 0;
+  while (b < 30 && (1 << (b + 1)) <= n) b++;
+  return b;
+}
+
+}  // namespace
+
+// Launches the query in a (order unset).  CUKD_QUERY_SORT: Morton-ordered
+// launch.  CUKD_QUERY_AUTOTUNE: the first call per (dims, traversal, list,
+// k, log2 nq, forest) times workgroup sizes 64/128/256 with and without
+// ordering on these very inputs (every variant writes identical results),
+// synchronising the stream once, and caches the fastest; later calls use
+// the cache.  repeatable = false (multi-round lists) only reads the cache.
+// order_ws (nullable): caller-owned ordering scratch of order_ws_bytes.
+static int run_queries(QueryArgs a, int D, int traversal, int flags, const MemResource &mr,
+                       void *order_ws, size_t order_ws_bytes, bool repeatable,
+                       hipStream_t stream, const char *what)
+{
+  TuneVal cfg{128, (flags & CUKD_QUERY_SORT) ? 1 : 0};
+  bool tune = false;
+  TuneKey key;
+  memset(&key, 0, sizeof(key));
+  key.D = D;
+  key.trav = traversal;
+  key.list = a.list_kind;
+  key.k = a.k;
+  key.bucket = log2_bucket(a.nq);
+  key.forest = a.fq.seg_off != nullptr;
+  if (flags & CUKD_QUERY_AUTOTUNE) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) cfg = it->second;
+    else tune = repeatable;
+  }
+  void *otmp = nullptr;
+  bool owned = false;
+  if (cfg.sort || tune) {
+    const size_t need = query_order_temp_bytes(a.nq);
+    if (order_ws && order_ws_bytes >= need) {
+      otmp = order_ws;
+    } else {
+      if (mr.alloc(&otmp, need, stream) != 0) return CUKD_ERR_ALLOC;
+      owned = true;
+    }
+  }
+  hipError_t e = hipSuccess;
+  if (tune) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float best = 3.4e38f;
+    TuneVal bestv = cfg;
+    const int blocks[3] = {64, 128, 256};
+    for (int sort = 0; sort < 2 && e == hipSuccess; sort++)
+      for (int bi = 0; bi < 3 && e == hipSuccess; bi++) {
+        QueryArgs v = a;
+        v.block = blocks[bi];
+        v.stats = nullptr;  // statistics count the final launch only
+        for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
+          if (rep == 1) e = hipEventRecord(e0, stream);
+          v.order = sort ? make_query_order(otmp, a.queries, a.nq, D, stream) : nullptr;
+          if (e == hipSuccess) e = launch_query(D, traversal, v, stream);
+        }
+        if (e == hipSuccess) e = hipEventRecord(e1, stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && ms < best) {
+          best = ms;
+          bestv = TuneVal{blocks[bi], sort};
+        }
+      }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e == hipSuccess) {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune[key] = bestv;
+      cfg = bestv;
+    }
+  }
+  if (e == hipSuccess) {
+    a.block = cfg.block;
+    a.order = cfg.sort ? make_query_order(otmp, a.queries, a.nq, D, stream) : nullptr;
+    e = launch_query(D, traversal, a, stream);
+  }
+  if (owned) mr.release(otmp, stream);
+  if (e != hipSuccess) {
+    set_last_error("%s launch failed: %s", what, hipGetErrorString(e));
+    return CUKD_ERR_HIP;
+  }
+  return CUKD_OK;
+}
+
 static int check_query_common(int traversal, const void *d_tree, int32_t n,
                               const cukd_layout *layout, const void *d_world_bounds,
                               const float *d_queries, int32_t nq, int32_t *d_ids)
@@ -264,9 +389,6 @@ int cukd_fcp(int traversal, const void *d_tree, int32_t n, const cukd_layout *la
     return rc;
   if (nq == 0) return CUKD_OK;
   MemResource mr{mem};
-  void *otmp = nullptr;
-  if ((flags & CUKD_QUERY_SORT) && mr.alloc(&otmp, query_order_temp_bytes(nq), stream) != 0)
-    return CUKD_ERR_ALLOC;
   QueryArgs a;
   a.t = make_view(d_tree, n, layout);
   a.queries = d_queries;
@@ -276,20 +398,14 @@ int cukd_fcp(int traversal, const void *d_tree, int32_t n, const cukd_layout *la
   a.eps = eps;
   a.k = 1;
   a.list_kind = -1;
-  a.order = otmp ? make_query_order(otmp, d_queries, nq, layout->num_dims, stream) : nullptr;
+  a.order = nullptr;
   a.ids = d_ids;
   a.dist2 = d_dist2;
   a.ret = nullptr;
   a.la = ListArgs{nullptr, 0, 0};
   a.fq = ForestQ{nullptr, nullptr, 0, nullptr};
   a.stats = g_stats;
-  const hipError_t e = launch_query(layout->num_dims, traversal, a, stream);
-  if (otmp) mr.release(otmp, stream);
-  if (e != hipSuccess) {
-    set_last_error("fcp launch failed: %s", hipGetErrorString(e));
-    return CUKD_ERR_HIP;
-  }
-  return CUKD_OK;
+  return run_queries(a, layout->num_dims, traversal, flags, mr, nullptr, 0, true, stream, "fcp");
 }
 
 int cukd_knn(int traversal, int list_kind, int k, const void *d_tree, int32_t n,
@@ -309,9 +425,6 @@ int cukd_knn(int traversal, int list_kind, int k, const void *d_tree, int32_t n,
   if (nq > 0 && !d_dist2) { set_last_error("d_dist2 is null"); return CUKD_ERR_INVALID; }
   if (nq == 0) return CUKD_OK;
   MemResource mr{mem};
-  void *otmp = nullptr;
-  if ((flags & CUKD_QUERY_SORT) && mr.alloc(&otmp, query_order_temp_bytes(nq), stream) != 0)
-    return CUKD_ERR_ALLOC;
   QueryArgs a;
   a.t = make_view(d_tree, n, layout);
   a.queries = d_queries;
@@ -321,20 +434,14 @@ int cukd_knn(int traversal, int list_kind, int k, const void *d_tree, int32_t n,
   a.eps = 0.f;
   a.k = k;
   a.list_kind = list_kind;
-  a.order = otmp ? make_query_order(otmp, d_queries, nq, layout->num_dims, stream) : nullptr;
+  a.order = nullptr;
   a.ids = d_ids;
   a.dist2 = d_dist2;
   a.ret = d_max_radius2;
   a.la = ListArgs{nullptr, 0, 0};
   a.fq = ForestQ{nullptr, nullptr, 0, nullptr};
   a.stats = g_stats;
-  const hipError_t e = launch_query(layout->num_dims, traversal, a, stream);
-  if (otmp) mr.release(otmp, stream);
-  if (e != hipSuccess) {
-    set_last_error("knn launch failed: %s", hipGetErrorString(e));
-    return CUKD_ERR_HIP;
-  }
-  return CUKD_OK;
+  return run_queries(a, layout->num_dims, traversal, flags, mr, nullptr, 0, true, stream, "knn");
 }
 
 int cukd_knn_accumulate(int traversal, int k, const void *d_tree, int32_t n,
@@ -351,9 +458,6 @@ int cukd_knn_accumulate(int traversal, int k, const void *d_tree, int32_t n,
   if (k < 1 || k > 64) { set_last_error("k = %d not in 1..64", k); return CUKD_ERR_INVALID; }
   if (nq == 0) return CUKD_OK;
   MemResource mr{mem};
-  void *otmp = nullptr;
-  if ((flags & CUKD_QUERY_SORT) && mr.alloc(&otmp, query_order_temp_bytes(nq), stream) != 0)
-    return CUKD_ERR_ALLOC;
   QueryArgs a;
   a.t = make_view(d_tree, n, layout);
   a.queries = d_queries;
@@ -363,20 +467,16 @@ int cukd_knn_accumulate(int traversal, int k, const void *d_tree, int32_t n,
   a.eps = 0.f;
   a.k = k;
   a.list_kind = LIST_ACCUM;
-  a.order = otmp ? make_query_order(otmp, d_queries, nq, layout->num_dims, stream) : nullptr;
+  a.order = nullptr;
   a.ids = nullptr;
   a.dist2 = nullptr;
   a.ret = nullptr;
   a.la = ListArgs{d_lists, init ? 1 : 0, id_offset};
   a.fq = ForestQ{nullptr, nullptr, 0, nullptr};
   a.stats = g_stats;
-  const hipError_t e = launch_query(layout->num_dims, traversal, a, stream);
-  if (otmp) mr.release(otmp, stream);
-  if (e != hipSuccess) {
-    set_last_error("knn accumulate launch failed: %s", hipGetErrorString(e));
-    return CUKD_ERR_HIP;
-  }
-  return CUKD_OK;
+  // a round updates its lists in place, so it is never re-run for tuning
+  return run_queries(a, layout->num_dims, traversal, flags, mr, nullptr, 0, false, stream,
+                     "knn accumulate");
 }
 
 int cukd_knn_extract(const uint64_t *d_lists, int32_t nq, int k, int32_t *d_ids,
@@ -579,11 +679,27 @@ int cukd_forest_build(cukd_forest *forest, void *d_points, void *d_world_bounds,
   return forest_build_run(forest->plan, d_points, d_world_bounds, forest->ws, stream);
 }
 
+int cukd_forest_reserve_queries(cukd_forest *forest, int32_t max_queries, hipStream_t stream)
+{
+  g_last_error[0] = 0;
+  if (!forest || max_queries < 0) { set_last_error("bad argument"); return CUKD_ERR_INVALID; }
+  const size_t need = query_order_temp_bytes(max_queries);
+  if (need <= forest->order_ws_bytes) return CUKD_OK;
+  MemResource mr{forest->has_mem ? &forest->mem : nullptr};
+  if (forest->order_ws) mr.release(forest->order_ws, stream);
+  forest->order_ws = nullptr;
+  forest->order_ws_bytes = 0;
+  if (mr.alloc(&forest->order_ws, need, stream) != 0) return CUKD_ERR_ALLOC;
+  forest->order_ws_bytes = need;
+  return CUKD_OK;
+}
+
 int cukd_forest_destroy(cukd_forest *forest, hipStream_t stream)
 {
   g_last_error[0] = 0;
   if (!forest) return CUKD_OK;
   MemResource mr{forest->has_mem ? &forest->mem : nullptr};
+  if (forest->order_ws) mr.release(forest->order_ws, stream);
   mr.release(forest->ws, stream);
   delete forest;
   return CUKD_OK;
@@ -593,7 +709,7 @@ static int forest_query(int traversal, int list_kind, int k, const cukd_forest *
                         const void *d_points, const void *d_world_bounds,
                         const float *d_queries, int32_t nq, int32_t num_segments,
                         const int32_t *d_seg_offsets, const int32_t *d_seg_tree, float cutoff,
-                        float eps, int32_t *d_ids, float *d_dist2, float *d_ret,
+                        float eps, int32_t *d_ids, float *d_dist2, float *d_ret, int flags,
                         hipStream_t stream)
 {
   if (!forest) { set_last_error("forest is null"); return CUKD_ERR_INVALID; }
@@ -622,31 +738,28 @@ static int forest_query(int traversal, int list_kind, int k, const cukd_forest *
   a.la = ListArgs{nullptr, 0, 0};
   a.fq = ForestQ{d_seg_offsets, d_seg_tree, num_segments, forest->tree_off};
   a.stats = g_stats;
-  const hipError_t e = launch_query(L.num_dims, traversal, a, stream);
-  if (e != hipSuccess) {
-    set_last_error("forest query launch failed: %s", hipGetErrorString(e));
-    return CUKD_ERR_HIP;
-  }
-  return CUKD_OK;
+  MemResource mr{forest->has_mem ? &forest->mem : nullptr};
+  return run_queries(a, L.num_dims, traversal, flags, mr, forest->order_ws,
+                     forest->order_ws_bytes, true, stream, "forest query");
 }
 
 int cukd_forest_fcp(int traversal, const cukd_forest *forest, const void *d_points,
                     const void *d_world_bounds, const float *d_queries, int32_t nq,
                     int32_t num_segments, const int32_t *d_seg_offsets,
                     const int32_t *d_seg_tree, float cutoff, float eps, int32_t *d_ids,
-                    float *d_dist2, hipStream_t stream)
+                    float *d_dist2, int flags, hipStream_t stream)
 {
   g_last_error[0] = 0;
   return forest_query(traversal, -1, 1, forest, d_points, d_world_bounds, d_queries, nq,
                       num_segments, d_seg_offsets, d_seg_tree, cutoff, eps, d_ids, d_dist2,
-                      nullptr, stream);
+                      nullptr, flags, stream);
 }
 
 int cukd_forest_knn(int traversal, int list_kind, int k, const cukd_forest *forest,
                     const void *d_points, const void *d_world_bounds, const float *d_queries,
                     int32_t nq, int32_t num_segments, const int32_t *d_seg_offsets,
                     const int32_t *d_seg_tree, float cutoff, int32_t *d_ids, float *d_dist2,
-                    float *d_max_radius2, hipStream_t stream)
+                    float *d_max_radius2, int flags, hipStream_t stream)
 {
   g_last_error[0] = 0;
   if (k < 1 || k > 64) { set_last_error("k = %d not in 1..64", k); return CUKD_ERR_INVALID; }
@@ -657,7 +770,7 @@ int cukd_forest_knn(int traversal, int list_kind, int k, const cukd_forest *fore
   if (nq > 0 && !d_dist2) { set_last_error("d_dist2 is null"); return CUKD_ERR_INVALID; }
   return forest_query(traversal, list_kind, k, forest, d_points, d_world_bounds, d_queries, nq,
                       num_segments, d_seg_offsets, d_seg_tree, cutoff, 0.f, d_ids, d_dist2,
-                      d_max_radius2, stream);
+                      d_max_radius2, flags, stream);
 }
 
 }  // extern "C"

@@ -168,6 +168,10 @@ void scan_exclusive(const uint32_t *in, uint32_t *out, long long n, uint32_t *sc
                     hipStream_t s);
 
 // Memory resource used for builder temporaries.
+void clear_last_error();
+int forest_num_dims(const cukd_forest *f);
+int forest_num_trees(const cukd_forest *f);
+
 struct MemResource {
   const cukd_mem_resource *user;
   int alloc(void **p, size_t bytes, hipStream_t s) const;

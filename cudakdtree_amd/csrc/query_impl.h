@@ -218,7 +218,7 @@ __device__ inline int segment_of(const int32_t *seg_off, int nseg, int qi)
 // batch kernel: one query per lane
 // ---------------------------------------------------------------------------
 template <int D, int METHOD, class R>
-__global__ __launch_bounds__(128) void k_query(TreeView t, const float *queries, int nq,
+__global__ __launch_bounds__(256) void k_query(TreeView t, const float *queries, int nq,
                                                const float *world_bounds, float cutoff,
                                                float eps, int k, const int32_t *order,
                                                int32_t *ids, float *dist2, float *ret,
@@ -275,6 +275,7 @@ struct QueryArgs {
   ListArgs la;
   ForestQ fq;
   unsigned long long *stats;
+  int block = 128;  // workgroup size: 64, 128 or 256
 };
 
 // persistent multi-round lists (cukd_knn_accumulate)
@@ -283,7 +284,7 @@ enum { LIST_ACCUM = 100 };
 template <int D, int METHOD, class R>
 inline hipError_t launch_q(const QueryArgs &a, hipStream_t s)
 {
-  const int nt = 128;
+  const int nt = a.block;
   const int blocks = (a.nq + nt - 1) / nt;
   hipLaunchKernelGGL((k_query<D, METHOD, R>), dim3(blocks), dim3(nt), 0, s, a.t, a.queries,
                      a.nq, a.world_bounds, a.cutoff, a.eps, a.k, a.order, a.ids, a.dist2,

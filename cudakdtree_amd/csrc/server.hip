@@ -1,0 +1,422 @@
+// server.hip -- a serving loop over a resident forest: requests (a tree, a
+// batch of host queries, FCP or kNN-k) arrive at any time from any thread;
+// one worker thread batches whatever is queued into one launch per query
+// kind (continuous batching), bounded by a device-memory budget, and
+// pipelines two batches (the next is packed and launched before the
+// previous one is waited for).  No counterpart in the reference, whose
+// queries are per-thread device calls; every result equals the single-tree
+// cukd_fcp / cukd_knn result for the same queries.
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <string.h>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "cukd_common.h"
+
+namespace cukd_amd {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+inline double ms_between(Clock::time_point a, Clock::time_point b)
+{
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct Request {
+  uint64_t ticket;
+  int tree;
+  const float *q;
+  int nq;
+  int k;  // 0 = FCP
+  float cutoff;
+  int32_t *ids;
+  float *dist2;
+  Clock::time_point t_submit, t_start, t_done;
+};
+
+inline size_t request_bytes(int nq, int k, int D)
+{
+  const size_t kk = k > 0 ? (size_t)k : 1;
+  return (size_t)nq * (D * 4 + kk * 8) + 64;
+}
+
+inline size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// one in-flight batch
+struct Slot {
+  uint8_t *dev = nullptr;   // queries | segment tables | results
+  uint8_t *host = nullptr;  // pinned mirror
+  size_t bytes = 0;
+  hipEvent_t done = nullptr;
+  std::vector<Request *> reqs;
+  std::vector<size_t> ids_off, d2_off;  // per request: where its results are in the slot
+  size_t used = 0;
+  bool busy = false;
+};
+
+}  // namespace
+
+}  // namespace cukd_amd
+
+using namespace cukd_amd;
+
+struct cukd_server {
+  const cukd_forest *forest = nullptr;
+  const void *points = nullptr;
+  const void *world_bounds = nullptr;
+  int traversal = 0;
+  int D = 0;
+  int max_per_batch = 0;
+  int device = 0;
+  size_t slot_bytes = 0;
+  hipStream_t stream = nullptr;
+  Slot slot[2];
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request *> pending;
+  std::deque<cukd_completion> done;
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  int error = 0;
+  char error_msg[256] = {0};
+  cukd_server_stats stats{};
+  std::thread worker;
+};
+
+namespace {
+
+// Packs the requests of one batch into a slot and launches it: one H2D
+// copy, one forest query launch per (k, cutoff) class, one D2H copy of the
+// results, then an event.
+int launch_batch(cukd_server *S, Slot &sl)
+{
+  const int D = S->D;
+  // classes in order of first appearance; requests keep admission order
+  // inside a class
+  struct Cls { int k; float cutoff; std::vector<int> members; };
+  std::vector<Cls> cls;
+  for (int i = 0; i < (int)sl.reqs.size(); i++) {
+    const Request *r = sl.reqs[i];
+    int c = 0;
+    for (; c < (int)cls.size(); c++)
+      if (cls[c].k == r->k && memcmp(&cls[c].cutoff, &r->cutoff, 4) == 0) break;
+    if (c == (int)cls.size()) cls.push_back(Cls{r->k, r->cutoff, {}});
+    cls[c].members.push_back(i);
+  }
+  // layout: per class [queries][seg_off][seg_tree]; then per class results
+  struct Lay { size_t q, so, st, ids, d2; int nq, nseg; };
+  std::vector<Lay> lay(cls.size());
+  size_t o = 0;
+  for (size_t c = 0; c < cls.size(); c++) {
+    int nq = 0;
+    for (int i : cls[c].members) nq += sl.reqs[i]->nq;
+    lay[c].nq = nq;
+    lay[c].nseg = (int)cls[c].members.size();
+    lay[c].q = o; o = align_up(o + (size_t)nq * D * 4);
+    lay[c].so = o; o = align_up(o + (size_t)(lay[c].nseg + 1) * 4);
+    lay[c].st = o; o = align_up(o + (size_t)lay[c].nseg * 4);
+  }
+  const size_t res_begin = o;
+  sl.ids_off.assign(sl.reqs.size(), 0);
+  sl.d2_off.assign(sl.reqs.size(), 0);
+  for (size_t c = 0; c < cls.size(); c++) {
+    const size_t kk = cls[c].k > 0 ? (size_t)cls[c].k : 1;
+    lay[c].ids = o; o = align_up(o + (size_t)lay[c].nq * kk * 4);
+    lay[c].d2 = o; o = align_up(o + (size_t)lay[c].nq * kk * 4);
+  }
+  if (o > sl.bytes) {
+    snprintf(S->error_msg, sizeof(S->error_msg), "batch of %zu bytes exceeds the slot", o);
+    return CUKD_ERR_INVALID;
+  }
+  sl.used = o;
+  for (size_t c = 0; c < cls.size(); c++) {
+    float *hq = (float *)(sl.host + lay[c].q);
+    int32_t *so = (int32_t *)(sl.host + lay[c].so);
+    int32_t *stt = (int32_t *)(sl.host + lay[c].st);
+    int acc = 0, s = 0;
+    const size_t kk = cls[c].k > 0 ? (size_t)cls[c].k : 1;
+    for (int i : cls[c].members) {
+      const Request *r = sl.reqs[i];
+      memcpy(hq + (size_t)acc * D, r->q, (size_t)r->nq * D * 4);
+      so[s] = acc;
+      stt[s] = r->tree;
+      sl.ids_off[i] = lay[c].ids + (size_t)acc * kk * 4;
+      sl.d2_off[i] = lay[c].d2 + (size_t)acc * kk * 4;
+      acc += r->nq;
+      s++;
+    }
+    so[s] = acc;
+  }
+  hipError_t e = hipMemcpyAsync(sl.dev, sl.host, res_begin, hipMemcpyHostToDevice, S->stream);
+  for (size_t c = 0; c < cls.size() && e == hipSuccess; c++) {
+    const float *dq = (const float *)(sl.dev + lay[c].q);
+    const int32_t *dso = (const int32_t *)(sl.dev + lay[c].so);
+    const int32_t *dst = (const int32_t *)(sl.dev + lay[c].st);
+    int32_t *ids = (int32_t *)(sl.dev + lay[c].ids);
+    float *d2 = (float *)(sl.dev + lay[c].d2);
+    int rc;
+    if (cls[c].k == 0)
+      rc = cukd_forest_fcp(S->traversal, S->forest, S->points, S->world_bounds, dq, lay[c].nq,
+                           lay[c].nseg, dso, dst, cls[c].cutoff, 0.f, ids, d2, 0, S->stream);
+    else
+      rc = cukd_forest_knn(S->traversal, CUKD_LIST_FIXED, cls[c].k, S->forest, S->points,
+                           S->world_bounds, dq, lay[c].nq, lay[c].nseg, dso, dst, cls[c].cutoff,
+                           ids, d2, nullptr, 0, S->stream);
+    if (rc != CUKD_OK) {
+      snprintf(S->error_msg, sizeof(S->error_msg), "batch launch: %s", cukd_last_error());
+      return rc;
+    }
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(sl.host + res_begin, sl.dev + res_begin, o - res_begin,
+                       hipMemcpyDeviceToHost, S->stream);
+  if (e == hipSuccess) e = hipEventRecord(sl.done, S->stream);
+  if (e != hipSuccess) {
+    snprintf(S->error_msg, sizeof(S->error_msg), "batch copy: %s", hipGetErrorString(e));
+    return CUKD_ERR_HIP;
+  }
+  sl.busy = true;
+  S->stats.batches++;
+  return CUKD_OK;
+}
+
+// Waits for a slot's batch, copies every request's results to its caller
+// and records its completion.
+int finish_batch(cukd_server *S, Slot &sl)
+{
+  const hipError_t e = hipEventSynchronize(sl.done);
+  const Clock::time_point now = Clock::now();
+  const size_t nreq = sl.reqs.size();
+  for (size_t i = 0; i < nreq; i++) {
+    Request *r = sl.reqs[i];
+    const size_t kk = r->k > 0 ? (size_t)r->k : 1;
+    if (e == hipSuccess) {
+      memcpy(r->ids, sl.host + sl.ids_off[i], (size_t)r->nq * kk * 4);
+      if (r->dist2) memcpy(r->dist2, sl.host + sl.d2_off[i], (size_t)r->nq * kk * 4);
+    }
+    r->t_done = now;
+  }
+  std::lock_guard<std::mutex> lk(S->mu);
+  for (size_t i = 0; i < nreq; i++) {
+    Request *r = sl.reqs[i];
+    cukd_completion c;
+    c.ticket = r->ticket;
+    c.status = e == hipSuccess ? CUKD_OK : CUKD_ERR_HIP;
+    c.num_queries = r->nq;
+    c.queue_ms = (float)ms_between(r->t_submit, r->t_start);
+    c.total_ms = (float)ms_between(r->t_submit, r->t_done);
+    S->done.push_back(c);
+    S->stats.requests_done++;
+    S->stats.queries_done += (uint64_t)r->nq;
+    delete r;
+  }
+  sl.reqs.clear();
+  sl.busy = false;
+  if (e != hipSuccess && !S->error) {
+    S->error = CUKD_ERR_HIP;
+    snprintf(S->error_msg, sizeof(S->error_msg), "batch failed: %s", hipGetErrorString(e));
+  }
+  S->cv_done.notify_all();
+  return e == hipSuccess ? CUKD_OK : CUKD_ERR_HIP;
+}
+
+void worker_main(cukd_server *S)
+{
+  (void)hipSetDevice(S->device);
+  int cur = 0;
+  int inflight = -1;
+  for (;;) {
+    Slot &sl = S->slot[cur];
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      if (inflight < 0)
+        S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty(); });
+      if (S->pending.empty() && inflight < 0 && S->stop) break;
+      // admit requests in arrival order while they fit the slot
+      size_t bytes = 0;
+      const Clock::time_point now = Clock::now();
+      while (!S->pending.empty() && (S->max_per_batch <= 0 || (int)sl.reqs.size() < S->max_per_batch)) {
+        Request *r = S->pending.front();
+        const size_t b = request_bytes(r->nq, r->k, S->D) + 4 * 256;
+        if (!sl.reqs.empty() && bytes + b > S->slot_bytes) break;
+        bytes += b;
+        r->t_start = now;
+        sl.reqs.push_back(r);
+        S->pending.pop_front();
+      }
+    }
+    if (!sl.reqs.empty()) {
+      if (launch_batch(S, sl) != CUKD_OK) {
+        // fail every request of the batch
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (Request *r : sl.reqs) {
+          cukd_completion c;
+          c.ticket = r->ticket;
+          c.status = CUKD_ERR_HIP;
+          c.num_queries = r->nq;
+          c.queue_ms = c.total_ms = 0.f;
+          S->done.push_back(c);
+          delete r;
+        }
+        sl.reqs.clear();
+        if (!S->error) S->error = CUKD_ERR_HIP;
+        S->cv_done.notify_all();
+        continue;
+      }
+    }
+    if (inflight >= 0) finish_batch(S, S->slot[inflight]);
+    inflight = sl.busy ? cur : -1;
+    if (sl.busy) cur ^= 1;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cukd_server_create(cukd_server **server, const cukd_forest *forest, const void *d_points,
+                       const void *d_world_bounds, int traversal, size_t budget_bytes,
+                       int32_t max_requests_per_batch, hipStream_t stream)
+{
+  clear_last_error();
+  if (!server || !forest) { set_last_error("null argument"); return CUKD_ERR_INVALID; }
+  *server = nullptr;
+  if (traversal < CUKD_STACK_BASED || traversal > CUKD_CCT) {
+    set_last_error("unknown traversal %d", traversal);
+    return CUKD_ERR_INVALID;
+  }
+  if (traversal == CUKD_CCT && !d_world_bounds) {
+    set_last_error("cct traversal needs world bounds");
+    return CUKD_ERR_INVALID;
+  }
+  if (budget_bytes < (1u << 20)) { set_last_error("budget below 1 MiB"); return CUKD_ERR_INVALID; }
+  cukd_server *S = new cukd_server();
+  S->forest = forest;
+  S->points = d_points;
+  S->world_bounds = d_world_bounds;
+  S->traversal = traversal;
+  S->max_per_batch = max_requests_per_batch;
+  S->stream = stream;
+  S->D = forest_num_dims(forest);
+  if (hipGetDevice(&S->device) != hipSuccess) S->device = 0;
+  S->slot_bytes = budget_bytes / 2;
+  for (int i = 0; i < 2; i++) {
+    Slot &sl = S->slot[i];
+    sl.bytes = S->slot_bytes;
+    hipError_t e = hipMalloc((void **)&sl.dev, sl.bytes);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.host, sl.bytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      set_last_error("server set-up failed: %s", hipGetErrorString(e));
+      for (int j = 0; j <= i; j++) {
+        if (S->slot[j].dev) (void)hipFree(S->slot[j].dev);
+        if (S->slot[j].host) (void)hipHostFree(S->slot[j].host);
+        if (S->slot[j].done) (void)hipEventDestroy(S->slot[j].done);
+      }
+      delete S;
+      return CUKD_ERR_ALLOC;
+    }
+  }
+  S->worker = std::thread(worker_main, S);
+  *server = S;
+  return CUKD_OK;
+}
+
+int cukd_server_submit(cukd_server *S, int32_t tree, const float *h_queries, int32_t nq, int k,
+                       float cutoff, int32_t *h_ids, float *h_dist2, uint64_t *ticket)
+{
+  clear_last_error();
+  if (!S || !ticket) { set_last_error("null argument"); return CUKD_ERR_INVALID; }
+  if (tree < 0 || tree >= forest_num_trees(S->forest)) {
+    set_last_error("tree %d out of range", tree);
+    return CUKD_ERR_INVALID;
+  }
+  if (nq < 1 || !h_queries || !h_ids) { set_last_error("empty request"); return CUKD_ERR_INVALID; }
+  if (k < 0 || k > 64) { set_last_error("k = %d not in 0..64", k); return CUKD_ERR_INVALID; }
+  if (k > 0 && !h_dist2) { set_last_error("kNN needs h_dist2"); return CUKD_ERR_INVALID; }
+  if (request_bytes(nq, k, S->D) + 4 * 256 > S->slot_bytes) {
+    set_last_error("request of %d queries exceeds the server's memory budget", nq);
+    return CUKD_ERR_INVALID;
+  }
+  Request *r = new Request();
+  r->tree = tree;
+  r->q = h_queries;
+  r->nq = nq;
+  r->k = k;
+  r->cutoff = cutoff;
+  r->ids = h_ids;
+  r->dist2 = h_dist2;
+  r->t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stop) {
+      delete r;
+      set_last_error("server is shutting down");
+      return CUKD_ERR_INVALID;
+    }
+    r->ticket = S->next_ticket++;
+    *ticket = r->ticket;
+    S->pending.push_back(r);
+    S->stats.requests_submitted++;
+  }
+  S->cv_work.notify_one();
+  return CUKD_OK;
+}
+
+int cukd_server_poll(cukd_server *S, cukd_completion *out, int32_t max, int32_t *count,
+                     int32_t wait_for)
+{
+  clear_last_error();
+  if (!S || !count || (max > 0 && !out)) { set_last_error("null argument"); return CUKD_ERR_INVALID; }
+  std::unique_lock<std::mutex> lk(S->mu);
+  if (wait_for > 0)
+    S->cv_done.wait(lk, [&] { return (int)S->done.size() >= wait_for || S->error; });
+  int n = 0;
+  while (n < max && !S->done.empty()) {
+    out[n++] = S->done.front();
+    S->done.pop_front();
+  }
+  *count = n;
+  if (S->error) {
+    set_last_error("%s", S->error_msg);
+    return S->error;
+  }
+  return CUKD_OK;
+}
+
+int cukd_server_get_stats(cukd_server *S, cukd_server_stats *stats)
+{
+  if (!S || !stats) return CUKD_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(S->mu);
+  *stats = S->stats;
+  return CUKD_OK;
+}
+
+int cukd_server_destroy(cukd_server *S)
+{
+  clear_last_error();
+  if (!S) return CUKD_OK;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv_work.notify_all();
+  S->worker.join();  // drains every pending request first
+  for (int i = 0; i < 2; i++) {
+    (void)hipFree(S->slot[i].dev);
+    (void)hipHostFree(S->slot[i].host);
+    (void)hipEventDestroy(S->slot[i].done);
+  }
+  const int err = S->error;
+  delete S;
+  return err;
+}
+
+}  // extern "C"

@@ -80,6 +80,13 @@ typedef struct cukd_mem_resource {
 /* query flags */
 #define CUKD_QUERY_SORT 1 /* Morton-order the queries before traversal
                              (results are returned in input order)      */
+#define CUKD_QUERY_AUTOTUNE 2 /* first call per (dims, traversal, list, k,
+                             log2 nq): time workgroup sizes 64/128/256 with
+                             and without Morton ordering on the caller's
+                             inputs (synchronises the stream once), keep
+                             the fastest for later calls; results are the
+                             same for every choice.  Not for multi-round
+                             lists (they only use an already tuned choice) */
 
 /* Builds a left-balanced k-d tree in place over d_points[0..n).
  * Replaces template<data_t,traits> buildTree / buildTree_thrust /
@@ -239,20 +246,82 @@ int cukd_forest_destroy(cukd_forest *forest, hipStream_t stream);
  * d_seg_offsets[s] .. d_seg_offsets[s+1]) and runs against tree
  * d_seg_tree[s] (both device arrays; a tree may appear in several
  * segments).  d_world_bounds (per-tree boxes) is required for CUKD_CCT.
- * Output layout and meaning as cukd_fcp / cukd_knn. */
+ * Output layout and meaning as cukd_fcp / cukd_knn; flags as there
+ * (CUKD_QUERY_SORT orders the whole batch). */
 int cukd_forest_fcp(int traversal, const cukd_forest *forest,
                     const void *d_points, const void *d_world_bounds,
                     const float *d_queries, int32_t nq, int32_t num_segments,
                     const int32_t *d_seg_offsets, const int32_t *d_seg_tree,
                     float cutoff, float eps, int32_t *d_ids, float *d_dist2,
-                    hipStream_t stream);
+                    int flags, hipStream_t stream);
 int cukd_forest_knn(int traversal, int list_kind, int k,
                     const cukd_forest *forest, const void *d_points,
                     const void *d_world_bounds, const float *d_queries,
                     int32_t nq, int32_t num_segments,
                     const int32_t *d_seg_offsets, const int32_t *d_seg_tree,
                     float cutoff, int32_t *d_ids, float *d_dist2,
-                    float *d_max_radius2, hipStream_t stream);
+                    float *d_max_radius2, int flags, hipStream_t stream);
+
+/* Reserves the Morton-ordering scratch for forest queries of up to
+ * max_queries queries, so that CUKD_QUERY_SORT forest queries allocate
+ * nothing (e.g. inside a captured HIP graph). */
+int cukd_forest_reserve_queries(cukd_forest *forest, int32_t max_queries,
+                                hipStream_t stream);
+
+/* ---- serving loop over a resident forest ----
+ *
+ * No counterpart in the reference.  Requests (one tree, nq host queries,
+ * FCP or kNN-k) are submitted from any thread; a worker thread packs
+ * whatever is queued into one batch (admitted in arrival order while the
+ * batch fits half of budget_bytes of device memory, at most
+ * max_requests_per_batch requests, <= 0 for no limit), launches one
+ * forest query per (k, cutoff) class, copies results back and completes
+ * the requests; the next batch is launched before the previous one is
+ * waited for.  max_requests_per_batch = 1 serves each request with its own
+ * launches.  Results equal cukd_fcp / cukd_knn (fixed list) on the
+ * request's tree alone. */
+typedef struct cukd_server cukd_server;
+
+typedef struct cukd_completion {
+  uint64_t ticket;
+  int32_t status;       /* CUKD_OK or an error code */
+  int32_t num_queries;
+  float queue_ms;       /* submit -> admitted into a batch */
+  float total_ms;       /* submit -> results in the caller's buffers */
+} cukd_completion;
+
+typedef struct cukd_server_stats {
+  uint64_t requests_submitted;
+  uint64_t requests_done;
+  uint64_t queries_done;
+  uint64_t batches;
+} cukd_server_stats;
+
+/* d_points / d_world_bounds: the built forest's arrays (bounds required
+ * for CUKD_CCT).  Allocates two device batch buffers and their pinned host
+ * mirrors (budget_bytes in total) and starts the worker thread. */
+int cukd_server_create(cukd_server **server, const cukd_forest *forest,
+                       const void *d_points, const void *d_world_bounds,
+                       int traversal, size_t budget_bytes,
+                       int32_t max_requests_per_batch, hipStream_t stream);
+
+/* Queues a request.  k = 0: FCP (h_ids[nq], h_dist2[nq] nullable);
+ * k in 1..64: kNN (h_ids / h_dist2 nq*k, ascending).  h_queries (nq rows
+ * of num_dims floats) and the output buffers must stay valid until the
+ * request's completion is returned by cukd_server_poll. */
+int cukd_server_submit(cukd_server *server, int32_t tree,
+                       const float *h_queries, int32_t nq, int k, float cutoff,
+                       int32_t *h_ids, float *h_dist2, uint64_t *ticket);
+
+/* Returns up to max completions (first waiting until at least wait_for
+ * are available, if wait_for > 0). */
+int cukd_server_poll(cukd_server *server, cukd_completion *out, int32_t max,
+                     int32_t *count, int32_t wait_for);
+
+int cukd_server_get_stats(cukd_server *server, cukd_server_stats *stats);
+
+/* Serves every queued request, stops the worker and frees the buffers. */
+int cukd_server_destroy(cukd_server *server);
 
 /* Traversal statistics, replacing CUKD_STATS / cukd::g_traversalStats
  * (common.h:348-360): while d_counter is non-null, every query launch adds

@@ -1,0 +1,145 @@
+"""Serving loop (cukd_server_*): requests batched by the worker thread give
+exactly the results of the single-tree queries, whether a request is
+served alone, in a burst with others, or with one launch per request."""
+import threading
+
+import numpy as np
+import pytest
+import torch
+
+import cudakdtree_amd as ck
+from cudakdtree_amd import datagen
+from tests.test_gpu_forest import forest_points, log_uniform_sizes, offsets_of
+
+pytestmark = pytest.mark.gpu
+
+KINDS = [(0, np.inf), (1, np.inf), (4, np.inf), (16, np.inf), (4, 0.05), (0, 0.02)]
+
+
+@pytest.fixture(scope="module")
+def served_forest():
+    D = 3
+    sizes = log_uniform_sizes(40, 10, 200000, 51)
+    off = offsets_of(sizes)
+    lay = ck.float_points(D)
+    pts = torch.from_numpy(forest_points(sizes, D, 53, dist="gmm")).cuda()
+    f = ck.Forest(off, lay)
+    f.build(pts)
+    torch.cuda.synchronize()
+    yield f, pts, off, lay
+    f.close()
+
+
+def _requests(off, count, seed):
+    T = len(off) - 1
+    r = datagen.splitmix64(seed, 4 * count)
+    reqs = []
+    for i in range(count):
+        tree = int(r[4 * i] % np.uint64(T))
+        u = float(r[4 * i + 1] >> np.uint64(11)) / float(1 << 53)
+        nq = int(np.exp(u * np.log(20000)))  # 1 .. 20000, log-uniform
+        k, cutoff = KINDS[int(r[4 * i + 2] % np.uint64(len(KINDS)))]
+        q = datagen.gmm(nq, 3, seed * 1000 + i)
+        reqs.append((tree, q, k, cutoff))
+    return reqs
+
+
+def _expected(pts, off, lay, tree, q, k, cutoff):
+    t = pts[off[tree]:off[tree + 1]]
+    dq = torch.from_numpy(q).cuda()
+    if k == 0:
+        ids, d2 = ck.fcp(t, lay, dq, cutoff=cutoff)
+    else:
+        ids, d2, _ = ck.knn(t, lay, dq, k, cutoff=cutoff)
+    return ids.cpu().numpy(), d2.cpu().numpy()
+
+
+def _serve(srv, reqs, one_at_a_time=False):
+    tickets = []
+    for tree, q, k, cutoff in reqs:
+        tickets.append(srv.submit(tree, q, k, cutoff))
+        if one_at_a_time:
+            done = srv.poll(wait_for=1)
+            assert len(done) == 1 and done[0].ticket == tickets[-1] and done[0].status == 0
+    pending = set(tickets) if not one_at_a_time else set()
+    while pending:
+        for c in srv.poll(wait_for=1):
+            assert c.status == 0
+            pending.discard(c.ticket)
+    return [srv.result(t) for t in tickets]
+
+
+def test_burst_matches_single_tree_queries(served_forest):
+    f, pts, off, lay = served_forest
+    reqs = _requests(off, 60, 7)
+    srv = ck.Server(f, pts, budget_bytes=64 << 20)
+    got = _serve(srv, reqs)
+    st = srv.stats()
+    srv.close()
+    assert st.requests_done == len(reqs) and st.batches < len(reqs)
+    for (tree, q, k, cutoff), (ids, d2) in zip(reqs, got):
+        eids, ed2 = _expected(pts, off, lay, tree, q, k, cutoff)
+        assert np.array_equal(ids, eids) and np.array_equal(d2.view(np.uint32), ed2.view(np.uint32))
+
+
+def test_same_result_alone_and_in_the_mix(served_forest):
+    f, pts, off, lay = served_forest
+    reqs = _requests(off, 40, 9)
+    srv = ck.Server(f, pts, budget_bytes=32 << 20)
+    mixed = _serve(srv, reqs)
+    alone = _serve(srv, reqs, one_at_a_time=True)
+    srv.close()
+    per_request = ck.Server(f, pts, budget_bytes=32 << 20, max_requests_per_batch=1)
+    single = _serve(per_request, reqs)
+    st = per_request.stats()
+    per_request.close()
+    assert st.batches == len(reqs)
+    for a, b, c in zip(mixed, alone, single):
+        for x, y, z in zip(a, b, c):
+            assert np.array_equal(x, y) and np.array_equal(x, z)
+
+
+def test_concurrent_submitters(served_forest):
+    f, pts, off, lay = served_forest
+    srv = ck.Server(f, pts, budget_bytes=64 << 20)
+    all_reqs = [_requests(off, 15, 100 + t) for t in range(4)]
+    results = [None] * 4
+    lock = threading.Lock()
+
+    def run(t):
+        tickets = []
+        for tree, q, k, cutoff in all_reqs[t]:
+            with lock:
+                tickets.append(srv.submit(tree, q, k, cutoff))
+        results[t] = tickets
+
+    th = [threading.Thread(target=run, args=(t,)) for t in range(4)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    want = sum(len(r) for r in all_reqs)
+    seen = 0
+    while seen < want:
+        seen += len(srv.poll(wait_for=1))
+    for t in range(4):
+        for (tree, q, k, cutoff), ticket in zip(all_reqs[t], results[t]):
+            ids, d2 = srv.result(ticket)
+            eids, ed2 = _expected(pts, off, lay, tree, q, k, cutoff)
+            assert np.array_equal(ids, eids) and np.array_equal(d2, ed2)
+    srv.close()
+
+
+def test_budget_and_argument_errors(served_forest):
+    f, pts, off, lay = served_forest
+    srv = ck.Server(f, pts, budget_bytes=1 << 20)
+    with pytest.raises(ck.CukdError):
+        srv.submit(0, datagen.uniform(100000, 3, 1), 16)  # larger than half the budget
+    with pytest.raises(ck.CukdError):
+        srv.submit(len(off) - 1, datagen.uniform(10, 3, 1), 4)  # no such tree
+    with pytest.raises(ck.CukdError):
+        srv.submit(0, datagen.uniform(10, 3, 1), 65)
+    # pending work is served before close returns
+    tickets = [srv.submit(i % (len(off) - 1), datagen.uniform(500, 3, i), 4) for i in range(20)]
+    srv.close()
+    assert all(t in srv._keep for t in tickets)
