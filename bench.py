@@ -129,6 +129,120 @@ def forest_bench(args, ck, datagen, torch, np):
     return out
 
 
+def serve_bench(args, ck, datagen, torch, np):
+    """Serving loop over resident trees: 512 float3 trees, sizes log-uniform
+    in [10K, 2M] (about 190M points), each a 64-centre Gaussian mixture in
+    its own random box.  Each request names a tree (uniform) and carries
+    1..100K (log-uniform) queries -- data points of that tree plus a small
+    Gaussian jitter -- for FCP or kNN with k in {1, 4, 16}.  The server's
+    saturation rate is measured first (all requests of a burst submitted at
+    once); requests then arrive as a Poisson process at 60% of it, and the
+    same arrival trace is replayed against the batching server and against
+    one launch sequence per request.  Latency = submit -> results in the
+    caller's host buffers."""
+    dev = torch.device("cuda:0")
+    T = args.serve_trees
+    r = (datagen.splitmix64(71, T) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    sizes = np.exp(np.log(10_000) + r * (np.log(2_000_000) - np.log(10_000))).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    N = int(off[-1])
+    g = torch.Generator(device=dev)
+    g.manual_seed(73)
+    lo = torch.rand((T, 3), generator=g, device=dev) * 100.0
+    ext = torch.rand((T, 3), generator=g, device=dev) * 10.0 + 0.1
+    centres = torch.rand((T, 64, 3), generator=g, device=dev) * ext[:, None, :] + lo[:, None, :]
+    tid = torch.repeat_interleave(torch.arange(T, device=dev), torch.from_numpy(sizes).to(dev))
+    cid = torch.randint(0, 64, (N,), generator=g, device=dev)
+    pts = (centres[tid, cid] + torch.randn((N, 3), generator=g, device=dev)
+           * (0.01 * ext[tid])).contiguous()
+    del tid, cid
+    # the request trace: tree, size, kind; queries = jittered data points
+    R = args.serve_requests
+    rr = datagen.splitmix64(79, 4 * R)
+    u = (rr >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    expo = -np.log(1.0 - u[0::4])
+    trees = (rr[1::4] % np.uint64(T)).astype(np.int64)
+    nqs = np.exp(u[2::4] * np.log(100_000)).astype(np.int64).clip(1, 100_000)
+    kinds = [0, 1, 4, 16]
+    ks = np.array([kinds[int(x)] for x in (rr[3::4] % np.uint64(4))])
+    qtree = torch.from_numpy(np.repeat(trees, nqs)).to(dev)
+    tsz = torch.from_numpy(sizes).to(dev)[qtree]
+    pick = torch.minimum((torch.rand(int(nqs.sum()), generator=g, device=dev) * tsz).long(),
+                         tsz - 1)
+    del tsz
+    qidx = torch.from_numpy(off[:-1]).to(dev)[qtree] + pick
+    qall = (pts[qidx] + torch.randn((len(qidx), 3), generator=g, device=dev)
+            * (0.001 * ext[qtree])).cpu().numpy()
+    del qtree, pick, qidx
+    qoff = np.concatenate([[0], np.cumsum(nqs)])
+    reqs = [(int(trees[i]), qall[qoff[i]:qoff[i + 1]], int(ks[i])) for i in range(R)]
+    lay = ck.float_points(3)
+    f = ck.Forest(off, lay)
+    f.build(pts)
+    torch.cuda.synchronize()
+
+    def replay(max_per_batch, arrivals, count):
+        srv = ck.Server(f, pts, budget_bytes=args.serve_budget_mb << 20,
+                        max_requests_per_batch=max_per_batch)
+        w = srv.submit(0, reqs[0][1][:16], 4)  # warm-up
+        srv.poll(wait_for=1)
+        srv.result(w)
+        tickets = []
+        t0 = time.perf_counter()
+        for i in range(count):
+            tree, q, k = reqs[i]
+            if arrivals is not None:
+                target = t0 + arrivals[i]
+                while time.perf_counter() < target:
+                    pass
+            tickets.append(srv.submit(tree, q, k))
+        lat = {}
+        while len(lat) < count:
+            for c in srv.poll(wait_for=1):
+                assert c.status == 0
+                lat[c.ticket] = c.total_ms
+        wall = time.perf_counter() - t0
+        res = [srv.result(t) for t in tickets]
+        st = srv.stats()
+        srv.close()
+        return np.array([lat[t] for t in tickets]), wall, res, int(st.batches)
+
+    # saturation: every request of a burst queued at once
+    nb = min(R, 1000)
+    _, burst_wall, _, _ = replay(0, None, nb)
+    cap_qps = float(nqs[:nb].sum()) / burst_wall
+    rate = 0.6 * cap_qps / float(nqs.mean())
+    arrivals = np.cumsum(expo / rate)
+    b_ms, b_wall, b_res, b_batches = replay(0, arrivals, R)
+    # one launch sequence per request, same arrivals (first second of them)
+    npr = int(min(R, max(50, rate * 1.0)))
+    p_ms, p_wall, p_res, p_batches = replay(1, arrivals, npr)
+    same = all(np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
+               for a, c in zip(b_res[:npr], p_res))
+    out = {"serve_trees": T, "serve_points": N, "serve_requests": R,
+           "serve_queries": int(nqs.sum()),
+           "serve_saturation_queries_per_s": round(cap_qps, 1),
+           "serve_rate_req_per_s": round(rate, 1),
+           "serve_p50_ms": round(float(np.percentile(b_ms, 50)), 3),
+           "serve_p99_ms": round(float(np.percentile(b_ms, 99)), 3),
+           "serve_queries_per_s": round(float(nqs.sum()) / b_wall, 1),
+           "serve_batches": b_batches,
+           "per_request_requests": npr,
+           "per_request_p50_ms": round(float(np.percentile(p_ms, 50)), 3),
+           "per_request_p99_ms": round(float(np.percentile(p_ms, 99)), 3),
+           "per_request_queries_per_s": round(float(nqs[:npr].sum()) / p_wall, 1),
+           "serve_equals_per_request": bool(same)}
+    print("[bench] serve: saturation %.1fM q/s, rate %.0f req/s; batched p50 %.2f ms p99 %.2f ms "
+          "(%d batches); per-request p50 %.2f ms p99 %.2f ms; identical: %s"
+          % (cap_qps / 1e6, rate, out["serve_p50_ms"], out["serve_p99_ms"], b_batches,
+             out["per_request_p50_ms"], out["per_request_p99_ms"], same),
+          file=sys.stderr, flush=True)
+    f.close()
+    del pts
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +259,10 @@ def main():
     ap.add_argument("--forest-trees", type=int, default=4096)
     ap.add_argument("--forest-steps", type=int, default=2)
     ap.add_argument("--no-forest", action="store_true")
+    ap.add_argument("--serve-trees", type=int, default=512)
+    ap.add_argument("--serve-requests", type=int, default=6000)
+    ap.add_argument("--serve-budget-mb", type=int, default=512)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--probe", action="store_true",
                     help="for hardware-counter runs: one 1 GiB device copy (the counters' "
                          "calibration), one build, one kNN and one FCP launch, no JSON line")
@@ -188,8 +306,11 @@ def main():
         ck.buildTree(tree, lay)
         ids = torch.empty((nq, args.k), dtype=torch.int32, device=dev)
         d2 = torch.empty((nq, args.k), dtype=torch.float32, device=dev)
-        ck.knn(tree, lay, queries, args.k, out=(ids, d2, None), flags=ck.QUERY_SORT)
-        ck.fcp(tree, lay, queries, flags=ck.QUERY_SORT)
+        # the queries in their Morton launch order, so that the kNN and FCP
+        # dispatches are exactly the ones the roofline figure times
+        qs = queries[ck.query_order(queries).long()].contiguous()
+        ck.knn(tree, lay, qs, args.k, out=(ids, d2, None))
+        ck.fcp(tree, lay, qs)
         torch.cuda.synchronize()
         print("[bench] probe done", file=sys.stderr, flush=True)
         return
@@ -277,9 +398,23 @@ def main():
     ck.set_traversal_stats(None)
     visits = int(counter.item())
     knn_bytes = nq * (S + k * 8 + 4) + visits * S
-    knn_gbs = knn_bytes / (ev_ms / 1e3) / 1e9
-    print("[bench] knn k=%d: %.3f ms/step (events %.3f ms), visits %.1f/query"
-          % (k, ms_per_step, ev_ms, visits / nq), file=sys.stderr, flush=True)
+    # the kNN kernel alone: the same queries already in the Morton order the
+    # step launches them in, event-timed on the launch stream
+    qs = queries[ck.query_order(queries).long()].contiguous()
+    for _ in range(args.warmup):
+        ck.knn(tree, lay, qs, k, out=out)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        ck.knn(tree, lay, qs, k, out=out)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    kernel_ms = e0.elapsed_time(e1) / args.steps
+    del qs
+    knn_gbs = knn_bytes / (kernel_ms / 1e3) / 1e9
+    print("[bench] knn k=%d: %.3f ms/step (events %.3f ms, kernel alone %.3f ms), visits "
+          "%.1f/query" % (k, ms_per_step, ev_ms, kernel_ms, visits / nq),
+          file=sys.stderr, flush=True)
 
     # FCP rate on the same tree
     fids = torch.empty(nq, dtype=torch.int32, device=dev)
@@ -295,6 +430,7 @@ def main():
 
     # ---------------- batched forest vs plain per-tree launches ----------------
     forest = None if args.no_forest else forest_bench(args, ck, datagen, torch, np)
+    serve = None if args.no_serve else serve_bench(args, ck, datagen, torch, np)
 
     # ---------------- CPU baseline (oracle, host cores) ----------------
     cpu = None
@@ -319,6 +455,16 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             cpu = {"value": None, "unit": "queries/s", "cores": 0, "kind": "port",
                    "sample": "failed: %s" % e}
+
+    # HBM bytes per kNN launch from the committed hardware-counter summary of
+    # this same workload (FETCH_SIZE + WRITE_SIZE, each scaled by what the
+    # counter reads for a 1 GiB device copy)
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "pmc_knn_10M_float3_k8.json")) as fh:
+            traffic = json.load(fh)["traffic_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        pass
 
     line = {
         "metric": "build ms @10M float3; kNN(k=8) queries/s; batched-vs-plain speedup; p99 ms",
@@ -347,16 +493,19 @@ def main():
             "knn_k8_queries_per_s_unordered": round(nq / (plain_ms / 1e3), 1),
             "fcp_queries_per_s": round(nq / (fcp_ms / 1e3), 1),
             "batched_vs_plain_speedup": forest["batched_vs_plain_speedup"] if forest else None,
-            "p99_ms": None,
+            "p99_ms": serve["serve_p99_ms"] if serve else None,
             "visits_per_query": round(visits / nq, 2),
             "forest": forest,
+            "serve": serve,
         },
         "roofline": {"bound": "hbm", "achieved": round(knn_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(knn_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": None,
-                     "kernel": "k_query<3, stack-based, FixedCandidateList<8>> + Morton ordering",
+                     "traffic": traffic,
+                     "kernel": "k_query<3, stack-based, FixedCandidateList<8>> on Morton-ordered "
+                               "queries",
                      "algorithmic_bytes_per_launch": knn_bytes,
-                     "avg_launch_ms": round(ev_ms, 4)},
+                     "avg_launch_ms": round(kernel_ms, 4),
+                     "step_ms_with_ordering": round(ev_ms, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = (
     "cukd_build_tree", "cukd_build_tree_temp_bytes", "cukd_compute_bounds",
     "cukd_fcp", "cukd_knn", "cukd_spatial_build", "cukd_spatial_free",
     "cukd_spatial_fcp", "cukd_spatial_knn", "cukd_set_traversal_stats",
-    "cukd_knn_accumulate", "cukd_knn_extract",
+    "cukd_knn_accumulate", "cukd_knn_extract", "cukd_query_order",
     "cukd_forest_create", "cukd_forest_workspace_bytes", "cukd_forest_build",
     "cukd_forest_destroy", "cukd_forest_fcp", "cukd_forest_knn", "cukd_forest_reserve_queries",
     "cukd_server_create", "cukd_server_submit", "cukd_server_poll",
@@ -154,6 +154,7 @@ def lib():
     L.cukd_knn_accumulate.argtypes = [ctypes.c_int, ctypes.c_int, P, i32, LP, P, P, i32, P,
                                        ctypes.c_int, f32, i32, ctypes.c_int, P, P]
     L.cukd_knn_extract.argtypes = [P, i32, ctypes.c_int, P, P, P, P]
+    L.cukd_query_order.argtypes = [P, i32, i32, P, P, P]
     L.cukd_forest_create.argtypes = [ctypes.POINTER(P), P, i32, LP, P, P]
     L.cukd_forest_workspace_bytes.argtypes = [P, ctypes.POINTER(ctypes.c_size_t)]
     L.cukd_forest_build.argtypes = [P, P, P, P]
@@ -297,6 +298,15 @@ def knn(tree, layout, queries, k, traversal=STACK_BASED, list_kind=LIST_FIXED,
                           _ptr(world_bounds), _ptr(queries), nq, cutoff, _ptr(ids),
                           _ptr(d2), _ptr(ret), flags, _stream(stream), None))
     return ids, d2, ret
+
+
+def query_order(queries, stream=None):
+    """The Morton launch order CUKD_QUERY_SORT uses (int32 permutation)."""
+    import torch
+    nq, D = queries.shape
+    order = torch.empty(nq, dtype=torch.int32, device=queries.device)
+    _check(lib().cukd_query_order(_ptr(queries), nq, D, _ptr(order), _stream(stream), None))
+    return order
 
 
 def knn_lists(nq, k, device="cuda"):

@@ -11,7 +11,8 @@ namespace cukd_amd {
 // held E = CAP/NT per thread, thread t owning positions t*E .. t*E+E-1;
 // sorts independent aligned blocks of size B ascending.  Strides below E
 // stay in registers, strides below 64*E use wave shuffles, larger ones go
-// through LDS.  Keys are unique apart from the ~0 padding.
+// through LDS, 32 bits at a time, so the sort needs only CAP words of LDS.
+// Keys are unique apart from the ~0 padding.
 template <int CAP, int NT>
 struct BlockSort {
   enum { E = CAP / NT };
@@ -42,8 +43,8 @@ struct BlockSort {
     }
   }
 
-  static __device__ inline void sort(uint64_t (&k)[E], uint32_t (&v)[E], int B, uint64_t *sx,
-                                     uint32_t *sv)
+  // sv: CAP 32-bit words of LDS for the cross-wave stages
+  static __device__ inline void sort(uint64_t (&k)[E], uint32_t (&v)[E], int B, uint32_t *sv)
   {
     const int base = threadIdx.x * E;
     for (int ks = 2; ks <= B; ks <<= 1) {
@@ -66,16 +67,34 @@ struct BlockSort {
             cmpx(k[e], v[e], ok, ov, ((i & j) == 0) == asc);
           }
         } else {
+          // cross-wave stage through one 32-bit LDS word per position: the
+          // key's high half, its low half, then the payload
+          uint64_t ok[E];
+          uint32_t ov[E];
 #pragma unroll
-          for (int e = 0; e < E; e++) { sx[base + e] = k[e]; sv[base + e] = v[e]; }
+          for (int e = 0; e < E; e++) sv[base + e] = (uint32_t)(k[e] >> 32);
+          __syncthreads();
+#pragma unroll
+          for (int e = 0; e < E; e++) ok[e] = (uint64_t)sv[(base + e) ^ j] << 32;
+          __syncthreads();
+#pragma unroll
+          for (int e = 0; e < E; e++) sv[base + e] = (uint32_t)k[e];
+          __syncthreads();
+#pragma unroll
+          for (int e = 0; e < E; e++) ok[e] |= sv[(base + e) ^ j];
+          __syncthreads();
+#pragma unroll
+          for (int e = 0; e < E; e++) sv[base + e] = v[e];
+          __syncthreads();
+#pragma unroll
+          for (int e = 0; e < E; e++) ov[e] = sv[(base + e) ^ j];
           __syncthreads();
 #pragma unroll
           for (int e = 0; e < E; e++) {
             const int i = base + e;
             const bool asc = (ks == B) || ((i & ks) == 0);
-            cmpx(k[e], v[e], sx[i ^ j], sv[i ^ j], ((i & j) == 0) == asc);
+            cmpx(k[e], v[e], ok[e], ov[e], ((i & j) == 0) == asc);
           }
-          __syncthreads();
         }
       }
     }

@@ -307,8 +307,8 @@ __device__ inline int tiles_of(const LevelGeom &g, int s, int T)
 // writes back and invalidates L2, which thrashes the cache for every other
 // tile): a tile adds its bins with value-returning atomics, so they have
 // been performed before the tile's barrier and before its thread 0 bumps
-// the subtree's tile counter; the tile that sees the final count reads and
-// clears the bins with atomic exchanges.
+// the subtree's tile counter; the tile that sees the final count reads the
+// bins with agent-scope loads (at L2) and clears the non-empty ones.
 __global__ __launch_bounds__(SEL_NT) void k_select_pass(
     TreeSet<LvlTree> set, int level, int word, int shift, int bits,
     const uint32_t *key, const uint8_t *data, const uint32_t *idx,
@@ -400,8 +400,14 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
     if (!s_last[sl]) continue;
     const int s = s_seg[sl];
     uint32_t *h = hist + (size_t)s * DIGIT_MAX_BINS;
-    // reuse s_hist[sl] for the subtree's global histogram (read + clear)
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[sl][b] = atomicExch(&h[b], 0u);
+    // reuse s_hist[sl] for the subtree's global histogram: agent-scope
+    // loads read the bins at L2, where every tile's flush has already been
+    // performed; only non-empty bins need clearing for the next pass
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
+      const uint32_t v = __hip_atomic_load(&h[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_hist[sl][b] = v;
+      if (v) __hip_atomic_store(&h[b], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
       SegState t = s_st[sl];
@@ -460,7 +466,7 @@ __global__ __launch_bounds__(SEL_NT) void k_partition(
 {
   __shared__ SegState s_st[2];
   __shared__ uint32_t s_cdim[4];
-  __shared__ uint64_t s_scan[SEL_NT];
+  __shared__ uint32_t s_cnt[SEL_ITEMS][SEL_NT / 64][4];
   __shared__ uint32_t s_base[4];
   __shared__ LvlTree s_tree;
   if (threadIdx.x == 0)
@@ -501,7 +507,6 @@ __global__ __launch_bounds__(SEL_NT) void k_partition(
   const int seg1_begin = seg1 != seg0 ? g.offset(seg1) : g.n_elems;
   // classify
   uint8_t cls[SEL_ITEMS]; // bit0-1: side (0 left, 1 right, 2 pivot), bit2: slot
-  uint64_t my = 0;        // 4 x 16-bit counters: (slot, side)
   for (int k = 0; k < SEL_ITEMS; k++) {
     const int p = t0 + k * SEL_NT + threadIdx.x;
     cls[k] = 0xff;
@@ -517,25 +522,35 @@ __global__ __launch_bounds__(SEL_NT) void k_partition(
       if (v != t.fixed[w]) { side = v < t.fixed[w] ? 0 : 1; break; }
     }
     cls[k] = (uint8_t)(side | (slot << 2));
-    if (side < 2) my += 1ull << (16 * (slot * 2 + side));
   }
-  // block exclusive scan of the packed counters
-  s_scan[threadIdx.x] = my;
-  __syncthreads();
-  for (int off = 1; off < SEL_NT; off <<= 1) {
-    const uint64_t v = threadIdx.x >= (unsigned)off ? s_scan[threadIdx.x - off] : 0ull;
-    __syncthreads();
-    s_scan[threadIdx.x] += v;
-    __syncthreads();
-  }
-  const uint64_t incl = s_scan[threadIdx.x];
-  uint64_t excl = incl - my;
-  if (threadIdx.x == SEL_NT - 1) {
-    for (int c = 0; c < 4; c++) {
-      const uint32_t total = (uint32_t)((incl >> (16 * c)) & 0xffffu);
-      const int s = (c >> 1) ? seg1 : seg0;
-      s_base[c] = total ? atomicAdd(&cursors[2 * s + (c & 1)], total) : 0u;
+  // Ranks inside each (slot, side) class in (item, wave, lane) order: for
+  // one item the lanes of a wave that go the same way write consecutive
+  // slots, so every store instruction is contiguous.  (Order inside a
+  // child is irrelevant: the next level selects by explicit key.)
+  const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint16_t wrank[SEL_ITEMS];
+  for (int k = 0; k < SEL_ITEMS; k++) {
+    const int c = (cls[k] == 0xff || (cls[k] & 3) == 2) ? -1 : ((cls[k] >> 2) * 2 + (cls[k] & 3));
+    wrank[k] = 0;
+    for (int cc = 0; cc < 4; cc++) {
+      const uint64_t m = __ballot(c == cc);
+      if (c == cc) wrank[k] = (uint16_t)__popcll(m & below);
+      if (lane == 0) s_cnt[k][wave][cc] = (uint32_t)__popcll(m);
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int cc = (int)threadIdx.x;
+    uint32_t run = 0;
+    for (int k = 0; k < SEL_ITEMS; k++)
+      for (int w = 0; w < SEL_NT / 64; w++) {
+        const uint32_t v = s_cnt[k][w][cc];
+        s_cnt[k][w][cc] = run;
+        run += v;
+      }
+    const int s = (cc >> 1) ? seg1 : seg0;
+    s_base[cc] = run ? atomicAdd(&cursors[2 * s + (cc & 1)], run) : 0u;
   }
   __syncthreads();
   for (int k = 0; k < SEL_ITEMS; k++) {
@@ -550,8 +565,7 @@ __global__ __launch_bounds__(SEL_NT) void k_partition(
       continue;
     }
     const int c = slot * 2 + side;
-    const uint32_t rank = (uint32_t)((excl >> (16 * c)) & 0xffffu);
-    excl += 1ull << (16 * c);
+    const uint32_t rank = s_cnt[k][wave][c] + wrank[k];
     const int dst = gn.offset(2 * s + side) + (int)(s_base[c] + rank);
     const int W = L.stride >> 2;
     uint32_t *dw = (uint32_t *)(ndata + (size_t)dst * L.stride);
@@ -572,7 +586,6 @@ __global__ __launch_bounds__(NT) void k_finish(
     const uint32_t *idxB, const SegState *st, uint8_t *out, cukd_layout L)
 {
   enum { E = CAP / NT };
-  __shared__ uint64_t s_x[CAP];
   __shared__ uint32_t s_xv[CAP];  // sort exchange, then the slot scatter
   __shared__ uint32_t s_split[EXPLICIT ? CAP : 1];
   __shared__ uint8_t s_sdim[EXPLICIT ? CAP : 1];
@@ -644,7 +657,7 @@ __global__ __launch_bounds__(NT) void k_finish(
         }
         key[e] = kk;
       }
-      BlockSort<CAP, NT>::sort(key, slot, Cs, s_x, s_xv);
+      BlockSort<CAP, NT>::sort(key, slot, Cs, s_xv);
       if (j == 0) break;
     }
   }
@@ -693,7 +706,7 @@ __global__ __launch_bounds__(NT) void k_finish(
       }
       key[e] = kk;
     }
-    if (B > 1) BlockSort<CAP, NT>::sort(key, slot, B, s_x, s_xv);
+    if (B > 1) BlockSort<CAP, NT>::sort(key, slot, B, s_xv);
 #pragma unroll
     for (int e = 0; e < E; e++) {
       if (key[e] == ~0ull) continue;
@@ -1002,7 +1015,7 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
       case 1: e = launch_finish<1024, 256>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
       case 2: e = launch_finish<2048, 512>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
       case 3: e = launch_finish<4096, 512>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
-      default: e = launch_finish<8192, 512>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
+      default: e = launch_finish<8192, 1024>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
     }
   }
   if (e != hipSuccess) {

@@ -274,7 +274,6 @@ __global__ __launch_bounds__(NT) void k_tl_finish(int n, int L0, uint8_t *data,
 {
   enum { E = CAP / NT };
   extern __shared__ uint32_t s_dyn[];  // CAP * W words of staged elements
-  __shared__ uint64_t s_x[CAP];
   __shared__ uint32_t s_xv[CAP];
   __shared__ uint32_t s_split[EXPLICIT ? CAP : 1];
   __shared__ uint8_t s_sdim[EXPLICIT ? CAP : 1];
@@ -347,7 +346,7 @@ __global__ __launch_bounds__(NT) void k_tl_finish(int n, int L0, uint8_t *data,
       }
       key[e] = kk;
     }
-    if (B > 1) BlockSort<CAP, NT>::sort(key, slot, B, s_x, s_xv);
+    if (B > 1) BlockSort<CAP, NT>::sort(key, slot, B, s_xv);
 #pragma unroll
     for (int e = 0; e < E; e++) {
       if (key[e] == ~0ull) continue;
@@ -384,7 +383,9 @@ int build_bounds_only(void *d_bounds, const void *d_points, int n, const cukd_la
 
 int finish_capacity_inplace(const cukd_layout &L)
 {
-  const int per = L.stride + 12 + (L.dim_offset >= 0 ? 6 : 0);
+  // LDS bytes per element: the staged element, the sort/scatter word, and
+  // for explicit dims the split value, split dim and block dim
+  const int per = L.stride + 4 + (L.dim_offset >= 0 ? 6 : 0);
   int cap = 4096;
   while (cap > 512 && cap * per > 150 * 1024) cap >>= 1;
   return cap;

@@ -479,6 +479,31 @@ int cukd_knn_accumulate(int traversal, int k, const void *d_tree, int32_t n,
                      "knn accumulate");
 }
 
+int cukd_query_order(const float *d_queries, int32_t nq, int32_t num_dims, int32_t *d_order,
+                     hipStream_t stream, const cukd_mem_resource *mem)
+{
+  g_last_error[0] = 0;
+  if (nq < 0 || num_dims < 1 || num_dims > CUKD_MAX_DIMS) {
+    set_last_error("bad query count or dims");
+    return CUKD_ERR_INVALID;
+  }
+  if (nq == 0) return CUKD_OK;
+  if (!d_queries || !d_order) { set_last_error("null pointer"); return CUKD_ERR_INVALID; }
+  MemResource mr{mem};
+  void *tmp = nullptr;
+  if (mr.alloc(&tmp, query_order_temp_bytes(nq), stream) != 0) return CUKD_ERR_ALLOC;
+  const int32_t *order = make_query_order(tmp, d_queries, nq, num_dims, stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_order, order, (size_t)nq * 4, hipMemcpyDeviceToDevice, stream);
+  mr.release(tmp, stream);
+  if (e != hipSuccess) {
+    set_last_error("query order failed: %s", hipGetErrorString(e));
+    return CUKD_ERR_HIP;
+  }
+  return CUKD_OK;
+}
+
 int cukd_knn_extract(const uint64_t *d_lists, int32_t nq, int k, int32_t *d_ids,
                      float *d_dist2, float *d_radius, hipStream_t stream)
 {

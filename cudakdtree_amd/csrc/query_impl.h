@@ -46,6 +46,18 @@ struct TreeView {
   }
 };
 
+// TreeView with the query's dimension count known at compile time: the
+// round-robin split dim of a node is then a constant-divisor modulo
+template <int D>
+struct TreeViewD : TreeView {
+  __device__ inline int dim(int i) const
+  {
+    if (dim_offset >= 0) return TreeView::dim(i);
+    const int l = 31 - __clz(i + 1);
+    return (D & (D - 1)) == 0 ? (l & (D - 1)) : l % D;
+  }
+};
+
 // FixedCandidateList with a run-time k (<= 64); entries live in scratch
 struct DynFixedList {
   uint64_t entry[64];
@@ -225,7 +237,14 @@ __global__ __launch_bounds__(256) void k_query(TreeView t, const float *queries,
                                                ListArgs la, ForestQ fq,
                                                unsigned long long *stats)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // Workgroups are dispatched round-robin over the 8 XCDs, each with its
+  // own L2: give every XCD one contiguous run of (Morton-ordered) queries,
+  // so neighbouring queries share an L2 instead of fetching the same tree
+  // nodes into eight of them.  A bijection of [0, gridDim.x).
+  const int nb = (int)gridDim.x;
+  const int xcd = (int)blockIdx.x & 7, q8 = nb >> 3, r8 = nb & 7;
+  const int lb = xcd * q8 + (xcd < r8 ? xcd : r8) + ((int)blockIdx.x >> 3);
+  const int i = lb * (int)blockDim.x + (int)threadIdx.x;
   if (i >= nq) return;
   const int qi = order ? order[i] : i;
   if (fq.seg_off) {
@@ -241,15 +260,17 @@ __global__ __launch_bounds__(256) void k_query(TreeView t, const float *queries,
   typename ResultOps<R>::State st;
   R &res = ResultOps<R>::init(st, cutoff, k, qi, la);
   unsigned visits = 0;
+  TreeViewD<D> tv;
+  static_cast<TreeView &>(tv) = t;
   if (METHOD == CUKD_STACK_BASED) {
-    cukd::detail::traverse_stack_based<D>(res, q, t, visits);
+    cukd::detail::traverse_stack_based<D>(res, q, tv, visits);
   } else if (METHOD == CUKD_STACK_FREE) {
-    cukd::detail::traverse_stack_free<D>(res, q, t, eps, visits);
+    cukd::detail::traverse_stack_free<D>(res, q, tv, eps, visits);
   } else {
     float lo[D], hi[D];
 #pragma unroll
     for (int d = 0; d < D; d++) { lo[d] = world_bounds[d]; hi[d] = world_bounds[D + d]; }
-    cukd::detail::traverse_cct<D>(res, q, lo, hi, t, visits);
+    cukd::detail::traverse_cct<D>(res, q, lo, hi, tv, visits);
   }
   ResultOps<R>::write(st, qi, k, ids, dist2, ret, la);
   // CUKD_STATS (common.h:348-360): one count per visited node

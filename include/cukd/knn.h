@@ -56,13 +56,14 @@ struct FixedCandidateList : public CandidateList<k> {
     // slot and fall off the end: skipping the network changes nothing
     // (keys are unique, ids being distinct).
     if (v >= entry[k - 1]) return;
+    // insert at the first slot holding a larger key, shifting the rest up
+    // by one (the same result as k compare-exchanges, one compare a slot)
+    bool gt[k];
 #pragma unroll
-    for (int i = 0; i < k; i++) {
-      const uint64_t lo = entry[i] < v ? entry[i] : v;
-      const uint64_t hi = entry[i] < v ? v : entry[i];
-      entry[i] = lo;
-      v = hi;
-    }
+    for (int i = 0; i < k; i++) gt[i] = entry[i] > v;
+#pragma unroll
+    for (int i = k - 1; i > 0; i--) entry[i] = gt[i - 1] ? entry[i - 1] : (gt[i] ? v : entry[i]);
+    entry[0] = gt[0] ? v : entry[0];
   }
   inline __both__ float processCandidate(int candPrimID, float candDist2)
   {

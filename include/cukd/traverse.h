@@ -22,13 +22,23 @@
 namespace cukd {
 namespace detail {
 
+// p[d] for a run-time d, as a blend of the D registers (a plain p[d]
+// or a select chain on floats gets turned into an indexed load, which
+// spills p to LDS on every visit)
 template <int D>
 inline __both__ float pick(const float (&p)[D], int d)
 {
-  float v = p[0];
+#ifdef __HIP_DEVICE_COMPILE__
+  unsigned v = __float_as_uint(p[0]);
 #pragma unroll
-  for (int i = 1; i < D; i++) v = d == i ? p[i] : v;
-  return v;
+  for (int i = 1; i < D; i++) {
+    const unsigned m = 0u - (unsigned)(d == i);
+    v = (__float_as_uint(p[i]) & m) | (v & ~m);
+  }
+  return __uint_as_float(v);
+#else
+  return p[d];
+#endif
 }
 
 template <int D>

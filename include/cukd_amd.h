@@ -147,6 +147,15 @@ int cukd_knn(int traversal, int list_kind, int k, const void *d_tree,
              float *d_max_radius2, int flags, hipStream_t stream,
              const cukd_mem_resource *mem);
 
+/* The launch order CUKD_QUERY_SORT uses: d_order (nq int32) receives a
+ * permutation of 0..nq-1 listing the queries by the Morton code of their
+ * coordinates (18-bit codes over the batch's bounding box; the order of
+ * queries inside one code cell is unspecified).  No counterpart in the
+ * reference. */
+int cukd_query_order(const float *d_queries, int32_t nq, int32_t num_dims,
+                     int32_t *d_order, hipStream_t stream,
+                     const cukd_mem_resource *mem);
+
 /* One round of a multi-round kNN: every query continues its own
  * FlexHeapCandidateList (knn.h:266-287, 360-421) held in HBM at
  * d_lists[q*k .. q*k+k) (uint64 entries, dist2 bits << 32 | id), as the

@@ -4,6 +4,8 @@
 // spatial tree.  Device results must equal the batched library results.
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "cukd/kdtree.h"
@@ -211,6 +213,80 @@ int main()
   }
   cukd::free(st);
   printf("spatial: nodes %d (bad %d)\n", st.numNodes, bad);
+
+  // 5. forest: three trees built together equal three buildTree calls, and
+  //    forest kNN equals per-tree batch kNN
+  {
+    const std::vector<int64_t> off = {0, 1000, 1000, n};  // one tree empty
+    float3 *d_f;
+    CHECK(hipMalloc(&d_f, n * sizeof(float3)));
+    CHECK(hipMemcpy(d_f, pts.data(), n * sizeof(float3), hipMemcpyHostToDevice));
+    CHECK(hipMemcpy(d_pts, pts.data(), n * sizeof(float3), hipMemcpyHostToDevice));
+    cukd::Forest<float3> forest(off);
+    forest.build(d_f);
+    cukd::buildTree(d_pts, 1000);
+    cukd::buildTree(d_pts + 1000, n - 1000);
+    CHECK(hipDeviceSynchronize());
+    std::vector<float3> a(n), b(n);
+    CHECK(hipMemcpy(a.data(), d_f, n * sizeof(float3), hipMemcpyDeviceToHost));
+    CHECK(hipMemcpy(b.data(), d_pts, n * sizeof(float3), hipMemcpyDeviceToHost));
+    int fb = memcmp(a.data(), b.data(), n * sizeof(float3)) != 0;
+    // queries: first half on tree 2, second half on tree 0
+    const int segOff[3] = {0, nq / 2, nq};
+    const int segTree[2] = {2, 0};
+    int *d_so, *d_st;
+    CHECK(hipMalloc(&d_so, sizeof(segOff)));
+    CHECK(hipMalloc(&d_st, sizeof(segTree)));
+    CHECK(hipMemcpy(d_so, segOff, sizeof(segOff), hipMemcpyHostToDevice));
+    CHECK(hipMemcpy(d_st, segTree, sizeof(segTree), hipMemcpyHostToDevice));
+    forest.knn(CUKD_STACK_BASED, CUKD_LIST_FIXED, 4, d_f, nullptr, d_q, nq, 2, d_so, d_st, d_a,
+               d_d2);
+    cukd::batch::knn<float3>(CUKD_STACK_BASED, CUKD_LIST_FIXED, 4, d_q, nq / 2, d_pts + 1000,
+                             n - 1000, nullptr, d_c, d_d2 + 4 * nq);
+    cukd::batch::knn<float3>(CUKD_STACK_BASED, CUKD_LIST_FIXED, 4, d_q + nq / 2, nq - nq / 2,
+                             d_pts, 1000, nullptr, d_c + 2 * nq, d_d2 + 4 * nq);
+    CHECK(hipDeviceSynchronize());
+    std::vector<int> x(4 * nq), y(4 * nq);
+    CHECK(hipMemcpy(x.data(), d_a, 4 * nq * sizeof(int), hipMemcpyDeviceToHost));
+    CHECK(hipMemcpy(y.data(), d_c, 4 * nq * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 4 * nq; i++) fb += x[i] != y[i];
+    bad += fb;
+    printf("forest: build and kNN equal per-tree calls (bad %d)\n", fb);
+
+    // 6. two accumulate rounds over the two halves of the points equal one
+    //    heap-list kNN over all of them (as sets; ids made global)
+    uint64_t *d_lists;
+    CHECK(hipMalloc(&d_lists, (size_t)nq * 8 * sizeof(uint64_t)));
+    cukd::knnAccumulate<float3>(CUKD_STACK_FREE, 8, d_f + 1000, n - 1000, nullptr, d_q, nq,
+                                d_lists, true, INFINITY, 1000);
+    cukd::knnAccumulate<float3>(CUKD_STACK_FREE, 8, d_f, 1000, nullptr, d_q, nq, d_lists, false);
+    float *d_r;
+    CHECK(hipMalloc(&d_r, nq * sizeof(float)));
+    cukd::knnExtract(d_lists, nq, 8, d_a, d_d2, d_r);
+    CHECK(hipDeviceSynchronize());
+    std::vector<float> d2a(8 * nq), rad(nq);
+    CHECK(hipMemcpy(d2a.data(), d_d2, 8 * nq * sizeof(float), hipMemcpyDeviceToHost));
+    CHECK(hipMemcpy(rad.data(), d_r, nq * sizeof(float), hipMemcpyDeviceToHost));
+    int mb = 0;
+    for (int i = 0; i < 64; i++) {
+      // brute force over the original points: the 8 smallest dist2
+      std::vector<float> all(n);
+      for (int j = 0; j < n; j++) {
+        float dx = pts[j].x - qs[i].x, dy = pts[j].y - qs[i].y, dz = pts[j].z - qs[i].z;
+        all[j] = dx * dx + dy * dy + dz * dz;
+      }
+      std::partial_sort(all.begin(), all.begin() + 8, all.end());
+      for (int j = 0; j < 8; j++) mb += all[j] != d2a[8 * i + j];
+      mb += rad[i] != sqrtf(all[7]);
+    }
+    bad += mb;
+    printf("multi-round: two rounds equal brute force (bad %d)\n", mb);
+    CHECK(hipFree(d_lists));
+    CHECK(hipFree(d_r));
+    CHECK(hipFree(d_f));
+    CHECK(hipFree(d_so));
+    CHECK(hipFree(d_st));
+  }
   printf(bad ? "FAILED\n" : "OK\n");
   return bad ? 1 : 0;
 }
