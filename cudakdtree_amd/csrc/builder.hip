@@ -37,7 +37,10 @@
 
 namespace cukd_amd {
 
-enum { SEL_NT = 256, SEL_ITEMS = 8, DIGIT_MAX_BINS = 2048 };
+enum { SEL_NT = 256, SEL_ITEMS = 8, SEL_BATCH = 8, DIGIT_MAX_BINS = 2048 };
+// per subtree: DIGIT_MAX_BINS counters, then one bit per bin that some tile
+// touched in the current pass
+enum { HIST_STRIDE = DIGIT_MAX_BINS + DIGIT_MAX_BINS / 32 };
 enum { MAX_WORDS = CUKD_MAX_DIMS + 1 };
 
 // Per-subtree selection state of one level.
@@ -307,8 +310,8 @@ __device__ inline int tiles_of(const LevelGeom &g, int s, int T)
 // writes back and invalidates L2, which thrashes the cache for every other
 // tile): a tile adds its bins with value-returning atomics, so they have
 // been performed before the tile's barrier and before its thread 0 bumps
-// the subtree's tile counter; the tile that sees the final count reads the
-// bins with agent-scope loads (at L2) and clears the non-empty ones.
+// the subtree's tile counter; the tile that sees the final count reads and
+// clears the touched bins with atomic exchanges.
 __global__ __launch_bounds__(SEL_NT) void k_select_pass(
     TreeSet<LvlTree> set, int level, int word, int shift, int bits,
     const uint32_t *key, const uint8_t *data, const uint32_t *idx,
@@ -321,6 +324,9 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
   __shared__ int s_seg[2];
   __shared__ int s_last[2];
   __shared__ uint32_t s_sink;
+  __shared__ uint32_t s_part[SEL_NT];
+  __shared__ uint32_t s_pick[3];
+  __shared__ uint32_t s_bm[DIGIT_MAX_BINS / 32];
   __shared__ LvlTree s_tree;
   if (threadIdx.x == 0)
     s_tree = find_tree(set, (int)blockIdx.x, [](const LvlTree &x) { return x.stile0; });
@@ -334,7 +340,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
   idx += tr.base;
   SegState *child_st = st + tr.cseg;
   st += tr.seg;
-  hist += (size_t)tr.hseg * DIGIT_MAX_BINS;
+  hist += (size_t)tr.hseg * HIST_STRIDE;
   const bool isf = L.scalar_type == CUKD_SCALAR_F32;
   const int nbins = 1 << bits;
   const int t0 = ((int)blockIdx.x - tr.stile0) * T;
@@ -349,36 +355,68 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
   const int seg1_begin = seg1 != seg0 ? g.offset(seg1) : g.n_elems;
   const uint32_t mask = (uint32_t)nbins - 1u;
   const int hi_shift = shift + bits;
-  for (int p = t0 + threadIdx.x; p < t1; p += blockDim.x) {
-    const int slot = p >= seg1_begin ? 1 : 0;
-    const SegState &t = s_st[slot];
-    if (t.resolved || (int)t.word != word) continue;
-    const uint32_t k0 = key[p];
-    uint32_t v;
-    if (word == 0) {
-      v = k0;
-    } else {
-      // candidates agree with the pivot on every earlier word
-      if (k0 != t.fixed[0]) continue;
-      const uint8_t *elem = data + (size_t)p * L.stride;
-      const uint32_t id = idx[p];
-      bool cand = true;
-      for (int w = 1; w < word && cand; w++)
-        cand = key_word(w, t, k0, elem, id, L, isf) == t.fixed[w];
-      if (!cand) continue;
-      v = key_word(word, t, k0, elem, id, L, isf);
+  // tiles whose subtrees are resolved or on another word only flush zeros
+  const bool act0 = !s_st[0].resolved && (int)s_st[0].word == word;
+  const bool act1 = seg1 != seg0 && !s_st[1].resolved && (int)s_st[1].word == word;
+  const int lane = (int)threadIdx.x & 63;
+  uint32_t *hflat = &s_hist[0][0];
+  for (int base = t0; (act0 || act1) && base < t1; base += SEL_NT * SEL_BATCH) {
+    // all of a thread's key loads in flight before any is used
+    uint32_t kv[SEL_BATCH];
+#pragma unroll
+    for (int u = 0; u < SEL_BATCH; u++) {
+      const int p = base + u * SEL_NT + (int)threadIdx.x;
+      kv[u] = p < t1 ? key[p] : 0u;
     }
-    if (hi_shift < 32 && (v >> hi_shift) != t.prefix) continue;
-    atomicAdd(&s_hist[slot][(v >> shift) & mask], 1u);
+#pragma unroll
+    for (int u = 0; u < SEL_BATCH; u++) {
+      const int p = base + u * SEL_NT + (int)threadIdx.x;
+      const int slot = p >= seg1_begin ? 1 : 0;
+      const SegState &t = s_st[slot];
+      bool has = p < t1 && (slot ? act1 : act0);
+      uint32_t v = kv[u];
+      if (has && word > 0) {
+        // candidates agree with the pivot on every earlier word
+        has = kv[u] == t.fixed[0];
+        if (has) {
+          const uint8_t *elem = data + (size_t)p * L.stride;
+          const uint32_t id = idx[p];
+          for (int w = 1; w < word && has; w++)
+            has = key_word(w, t, kv[u], elem, id, L, isf) == t.fixed[w];
+          if (has) v = key_word(word, t, kv[u], elem, id, L, isf);
+        }
+      }
+      if (has && hi_shift < 32 && (v >> hi_shift) != t.prefix) has = false;
+      const uint32_t hb = (uint32_t)slot * DIGIT_MAX_BINS + ((v >> shift) & mask);
+      // one LDS atomic per wave when every counted lane hits the same bin
+      // (the common case deep in the tree, where a subtree's keys share
+      // their high digits)
+      const uint64_t act = __ballot(has);
+      if (act) {
+        const int first = __ffsll((unsigned long long)act) - 1;
+        const uint32_t fb = __shfl(hb, first);
+        const uint64_t same = __ballot(has && hb == fb);
+        if (same == act) {
+          if (lane == first) atomicAdd(&hflat[fb], (uint32_t)__popcll(act));
+        } else if (has) {
+          atomicAdd(&hflat[hb], 1u);
+        }
+      }
+    }
   }
   __syncthreads();
   const int nslots = seg1 != seg0 ? 2 : 1;
   uint32_t sink = 0;
   for (int sl = 0; sl < nslots; sl++) {
     const int s = s_seg[sl];
+    uint32_t *h = hist + (size_t)s * HIST_STRIDE;
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
       const uint32_t c = s_hist[sl][b];
-      if (c) sink |= atomicAdd(&hist[(size_t)s * DIGIT_MAX_BINS + b], c);
+      if (c) sink |= atomicAdd(&h[b], c);
+      // bins b..: lanes 0-31 and 32-63 of a wave cover two bitmap words
+      const uint64_t m = __ballot(c != 0u);
+      const uint32_t part = (lane < 32) ? (uint32_t)m : (uint32_t)(m >> 32);
+      if ((lane & 31) == 0 && part) sink |= atomicOr(&h[DIGIT_MAX_BINS + (b >> 5)], part);
     }
   }
   // consuming the returned values makes every atomic complete before the
@@ -399,26 +437,56 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
   for (int sl = 0; sl < nslots; sl++) {
     if (!s_last[sl]) continue;
     const int s = s_seg[sl];
-    uint32_t *h = hist + (size_t)s * DIGIT_MAX_BINS;
-    // reuse s_hist[sl] for the subtree's global histogram: agent-scope
-    // loads read the bins at L2, where every tile's flush has already been
-    // performed; only non-empty bins need clearing for the next pass
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
-      const uint32_t v = __hip_atomic_load(&h[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s_hist[sl][b] = v;
-      if (v) __hip_atomic_store(&h[b], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    uint32_t *h = hist + (size_t)s * HIST_STRIDE;
+    // reuse s_hist[sl] for the subtree's global histogram.  Read-modify-
+    // write atomics are performed where every tile's flush was (plain or
+    // relaxed loads may be served by a stale copy in this XCD's L2), and
+    // the bitmap limits them to the bins some tile touched; reading with
+    // exchanges also clears the counters for the next pass.
+    for (int w = threadIdx.x; w < nbins / 32; w += blockDim.x) s_bm[w] = atomicExch(&h[DIGIT_MAX_BINS + w], 0u);
     __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+      s_hist[sl][b] = ((s_bm[b >> 5] >> (b & 31)) & 1u) ? atomicExch(&h[b], 0u) : 0u;
+    __syncthreads();
+    if (!s_st[sl].resolved && (int)s_st[sl].word == word) {
+      // the bin holding rank krem: per-thread sums of consecutive bins, an
+      // inclusive scan over the threads, then the owning thread walks its
+      // own few bins (a serial walk over all bins would cost a dependent
+      // LDS load per bin)
+      const int per = nbins / SEL_NT;
+      const int b0 = (int)threadIdx.x * per;
+      uint32_t mine = 0;
+      for (int i = 0; i < per; i++) mine += s_hist[sl][b0 + i];
+      s_part[threadIdx.x] = mine;
+      __syncthreads();
+      for (int off = 1; off < SEL_NT; off <<= 1) {
+        const uint32_t add = (int)threadIdx.x >= off ? s_part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s_part[threadIdx.x] += add;
+        __syncthreads();
+      }
+      const uint32_t incl = s_part[threadIdx.x];
+      const uint32_t krem = s_st[sl].krem;
+      if (krem >= incl - mine && krem < incl) {
+        uint32_t before = incl - mine, cnt = 0;
+        int b = b0;
+        for (int i = 0; i < per; i++) {
+          cnt = s_hist[sl][b0 + i];
+          b = b0 + i;
+          if (before + cnt > krem) break;
+          before += cnt;
+        }
+        s_pick[0] = (uint32_t)b;
+        s_pick[1] = before;
+        s_pick[2] = cnt;
+      }
+      __syncthreads();
+    }
     if (threadIdx.x == 0) {
       SegState t = s_st[sl];
       t.tiles_done = 0;
       if (!t.resolved && (int)t.word == word) {
-        uint32_t before = 0, b = 0, cnt = 0;
-        for (b = 0; b < (uint32_t)nbins; b++) {
-          cnt = s_hist[sl][b];
-          if (before + cnt > t.krem) break;
-          before += cnt;
-        }
+        const uint32_t b = s_pick[0], before = s_pick[1], cnt = s_pick[2];
         t.krem -= before;
         t.prefix = (hi_shift < 32 ? (t.prefix << bits) : 0u) | b;
         if (shift == 0) {
@@ -897,7 +965,7 @@ int make_forest_plan(ForestPlan &P, const long long *off, int ntrees, const cukd
     P.off_dataB = P.off_idxB = P.off_keyB = 0;
   }
   P.off_st = o; o = align_up(o + P.nstates * sizeof(SegState));
-  P.off_hist = o; o = align_up(o + P.max_hseg * DIGIT_MAX_BINS * 4);
+  P.off_hist = o; o = align_up(o + P.max_hseg * HIST_STRIDE * 4);
   P.off_cursors = o; o = align_up(o + P.max_hseg * 2 * 4);
   P.off_need = o; o = align_up(o + MAX_WORDS * 4);
   P.zero_end = o;

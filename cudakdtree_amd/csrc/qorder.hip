@@ -105,8 +105,9 @@ int32_t *make_query_order(void *tmp, const float *q, int nq, int D, hipStream_t 
   t += 2 * CUKD_MAX_DIMS * 4;
   uint32_t *scratch = (uint32_t *)t;
   hipLaunchKernelGGL(k_qbounds_init, dim3(1), dim3(64), 0, s, acc, D);
+  // few workgroups: each ends with 2*D atomics on the same 2*D words
   int blocks = (nq + 255) / 256;
-  if (blocks > 1024) blocks = 1024;
+  if (blocks > 128) blocks = 128;
   hipLaunchKernelGGL(k_qbounds, dim3(blocks), dim3(256), 0, s, acc, q, nq, D);
   hipLaunchKernelGGL(k_qzero, dim3(256), dim3(256), 0, s, hist, buckets);
   const int g = (nq + 255) / 256;

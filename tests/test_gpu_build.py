@@ -160,6 +160,26 @@ def test_compute_bounds():
     np.testing.assert_array_equal(b.cpu().numpy(), oc.compute_bounds(coords, lay))
 
 
+def test_ten_million_builds_repeat_and_agree():
+    """At the bench size (10M float3, 24-bit uniform coordinates, so ties
+    are common) the radix-select builder gives the same array on every run,
+    and that array equals the independent bitonic builder's; both builders
+    equal buildTree_host at every size the oracle checks quickly."""
+    n = 10_000_000
+    coords = datagen.uniform(n, 3, 1)
+    lay = ck.float_points(3)
+    src = torch.from_numpy(coords).cuda()
+    outs = []
+    for b in (ck.BUILDER_DEFAULT, ck.BUILDER_DEFAULT, ck.BUILDER_BITONIC, ck.BUILDER_DEFAULT,
+              ck.BUILDER_DEFAULT):
+        t = src.clone()
+        ck.build_tree(t, lay, b)
+        outs.append(t)
+    torch.cuda.synchronize()
+    for i, o in enumerate(outs[1:]):
+        assert torch.equal(o, outs[0]), "build %d differs" % (i + 1)
+
+
 @pytest.mark.slow
 def test_ten_million_properties():
     """Full bench size: a valid k-d tree and a permutation of the input
