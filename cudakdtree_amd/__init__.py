@@ -37,6 +37,7 @@ STACK_BASED, STACK_FREE, CCT = 0, 1, 2
 LIST_FIXED, LIST_HEAP = 0, 1
 QUERY_AUTOTUNE = 2
 QUERY_SORT = 1
+QUERY_ONE_PER_LANE = 4
 
 # every entry point declared in include/cukd_amd.h
 EXPORTED_SYMBOLS = (

@@ -238,7 +238,7 @@ struct TuneKey {
   }
 };
 struct TuneVal {
-  int block, sort;
+  int block, sort, lanes;
 };
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneVal> g_tune;
@@ -255,7 +255,8 @@ int log2_bucket(int n)
 // Launches the query in a (order unset).  CUKD_QUERY_SORT: Morton-ordered
 // launch.  CUKD_QUERY_AUTOTUNE: the first call per (dims, traversal, list,
 // k, log2 nq, forest) times workgroup sizes 64/128/256 with and without
-// ordering on these very inputs (every variant writes identical results),
+// ordering, and one or two queries per lane where the kernel allows two,
+// on these very inputs (every variant writes identical results),
 // synchronising the stream once, and caches the fastest; later calls use
 // the cache.  repeatable = false (multi-round lists) only reads the cache.
 // order_ws (nullable): caller-owned ordering scratch of order_ws_bytes.
@@ -263,7 +264,14 @@ static int run_queries(QueryArgs a, int D, int traversal, int flags, const MemRe
                        void *order_ws, size_t order_ws_bytes, bool repeatable,
                        hipStream_t stream, const char *what)
 {
-  TuneVal cfg{128, (flags & CUKD_QUERY_SORT) ? 1 : 0};
+  TuneVal cfg{128, (flags & CUKD_QUERY_SORT) ? 1 : 0, (flags & CUKD_QUERY_ONE_PER_LANE) ? 1 : 2};
+  // two queries per lane: stack-based, registers-only lists, round-robin
+  // dims, node offsets within 32 bits
+  const bool two_ok = traversal == CUKD_STACK_BASED && a.t.dim_offset < 0 &&
+                      (a.fq.seg_off ? a.tree_bytes : (uint64_t)a.t.n * (uint64_t)a.t.stride) <=
+                          0xffffffffull &&
+                      (a.list_kind < 0 || (a.list_kind == CUKD_LIST_FIXED &&
+                                           (a.k == 1 || a.k == 4 || a.k == 8 || a.k == 16)));
   bool tune = false;
   TuneKey key;
   memset(&key, 0, sizeof(key));
@@ -298,10 +306,12 @@ static int run_queries(QueryArgs a, int D, int traversal, int flags, const MemRe
     float best = 3.4e38f;
     TuneVal bestv = cfg;
     const int blocks[3] = {64, 128, 256};
+    for (int lanes = 1; lanes <= (two_ok ? 2 : 1) && e == hipSuccess; lanes++)
     for (int sort = 0; sort < 2 && e == hipSuccess; sort++)
       for (int bi = 0; bi < 3 && e == hipSuccess; bi++) {
         QueryArgs v = a;
         v.block = blocks[bi];
+        v.lanes = lanes;
         v.stats = nullptr;  // statistics count the final launch only
         for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
           if (rep == 1) e = hipEventRecord(e0, stream);
@@ -314,7 +324,7 @@ static int run_queries(QueryArgs a, int D, int traversal, int flags, const MemRe
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
         if (e == hipSuccess && ms < best) {
           best = ms;
-          bestv = TuneVal{blocks[bi], sort};
+          bestv = TuneVal{blocks[bi], sort, lanes};
         }
       }
     if (e0) (void)hipEventDestroy(e0);
@@ -327,6 +337,7 @@ static int run_queries(QueryArgs a, int D, int traversal, int flags, const MemRe
   }
   if (e == hipSuccess) {
     a.block = cfg.block;
+    a.lanes = two_ok ? cfg.lanes : 1;
     a.order = cfg.sort ? make_query_order(otmp, a.queries, a.nq, D, stream) : nullptr;
     e = launch_query(D, traversal, a, stream);
   }
@@ -762,6 +773,7 @@ static int forest_query(int traversal, int list_kind, int k, const cukd_forest *
   a.ret = d_ret;
   a.la = ListArgs{nullptr, 0, 0};
   a.fq = ForestQ{d_seg_offsets, d_seg_tree, num_segments, forest->tree_off};
+  a.tree_bytes = (uint64_t)forest->plan.ntotal * (uint64_t)L.stride;
   a.stats = g_stats;
   MemResource mr{forest->has_mem ? &forest->mem : nullptr};
   return run_queries(a, L.num_dims, traversal, flags, mr, forest->order_ws,

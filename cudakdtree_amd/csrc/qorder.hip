@@ -10,48 +10,88 @@
 
 namespace cukd_amd {
 
-enum { MORTON_BITS = 18 };
+enum { MORTON_BITS = 18, QB_SLOTS = 32, QB_NT = 256 };
 
-__global__ __launch_bounds__(256) void k_qbounds(uint32_t *acc, const float *q, int nq, int D)
+// Bounds accumulators: QB_SLOTS copies of (lo[CUKD_MAX_DIMS], hi[CUKD_MAX_DIMS])
+// as ordered keys; workgroup b folds into copy b % QB_SLOTS, so each word
+// takes few atomics, and readers fold the copies.
+enum { QB_WORDS = QB_SLOTS * 2 * CUKD_MAX_DIMS };
+
+// clears the histogram and sets the accumulators to empty bounds
+__global__ void k_qinit(uint32_t *hist, int buckets, uint32_t *acc)
 {
-  __shared__ uint32_t s_lo[CUKD_MAX_DIMS][256], s_hi[CUKD_MAX_DIMS][256];
-  uint32_t lo[CUKD_MAX_DIMS], hi[CUKD_MAX_DIMS];
-  for (int d = 0; d < D; d++) { lo[d] = 0xffffffffu; hi[d] = 0u; }
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x)
-    for (int d = 0; d < D; d++) {
-      const uint32_t k = ordered_key(__float_as_uint(q[(size_t)i * D + d]), true);
-      lo[d] = min(lo[d], k);
-      hi[d] = max(hi[d], k);
-    }
-  for (int d = 0; d < D; d++) { s_lo[d][threadIdx.x] = lo[d]; s_hi[d][threadIdx.x] = hi[d]; }
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < buckets; i += stride) hist[i] = 0u;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < QB_WORDS; i += stride)
+    acc[i] = (i % (2 * CUKD_MAX_DIMS)) < CUKD_MAX_DIMS ? 0xffffffffu : 0u;
+}
+
+// Per-dim min / max of the nq*D query coordinates.  The flat array is read
+// coalesced with a thread stride T that is a multiple of D, so each thread
+// sees one dim only; four loads are in flight per thread.
+__global__ __launch_bounds__(QB_NT) void k_qbounds(uint32_t *acc, const float *q, long long E, int D,
+                                                  long long T)
+{
+  __shared__ uint32_t s_lo[CUKD_MAX_DIMS], s_hi[CUKD_MAX_DIMS];
+  if (threadIdx.x < CUKD_MAX_DIMS) {
+    s_lo[threadIdx.x] = 0xffffffffu;
+    s_hi[threadIdx.x] = 0u;
+  }
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off)
-      for (int d = 0; d < D; d++) {
-        s_lo[d][threadIdx.x] = min(s_lo[d][threadIdx.x], s_lo[d][threadIdx.x + off]);
-        s_hi[d][threadIdx.x] = max(s_hi[d][threadIdx.x], s_hi[d][threadIdx.x + off]);
+  const long long tid = (long long)blockIdx.x * QB_NT + threadIdx.x;
+  if (tid < T) {
+    uint32_t lo = 0xffffffffu, hi = 0u;
+    for (long long e = tid; e < E; e += 4 * T) {
+      uint32_t v[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const long long x = e + j * T;
+        v[j] = x < E ? ordered_key(__float_as_uint(q[x]), true) : ordered_key(__float_as_uint(q[e]), true);
       }
-    __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        lo = min(lo, v[j]);
+        hi = max(hi, v[j]);
+      }
+    }
+    const int d = (int)(tid % D);
+    if (tid < E) {
+      atomicMin(&s_lo[d], lo);
+      atomicMax(&s_hi[d], hi);
+    }
   }
+  __syncthreads();
   if ((int)threadIdx.x < D) {
-    atomicMin(&acc[threadIdx.x], s_lo[threadIdx.x][0]);
-    atomicMax(&acc[D + threadIdx.x], s_hi[threadIdx.x][0]);
+    uint32_t *a = acc + (blockIdx.x % QB_SLOTS) * 2 * CUKD_MAX_DIMS;
+    if (s_lo[threadIdx.x] != 0xffffffffu) atomicMin(&a[threadIdx.x], s_lo[threadIdx.x]);
+    if (s_hi[threadIdx.x] != 0u) atomicMax(&a[CUKD_MAX_DIMS + threadIdx.x], s_hi[threadIdx.x]);
   }
 }
 
-__global__ void k_qbounds_init(uint32_t *acc, int D)
+// the batch box, folded from the accumulators (written by the previous
+// kernel, so plain loads see them)
+__device__ inline void qbox_load(const uint32_t *acc, int D, float *s_lo, float *s_hi)
 {
-  const int d = threadIdx.x;
-  if (d < D) { acc[d] = 0xffffffffu; acc[D + d] = 0u; }
+  if ((int)threadIdx.x < D) {
+    const int d = threadIdx.x;
+    uint32_t lo = 0xffffffffu, hi = 0u;
+    for (int s = 0; s < QB_SLOTS; s++) {
+      lo = min(lo, acc[s * 2 * CUKD_MAX_DIMS + d]);
+      hi = max(hi, acc[s * 2 * CUKD_MAX_DIMS + CUKD_MAX_DIMS + d]);
+    }
+    s_lo[d] = __uint_as_float(unordered_key(lo, true));
+    s_hi[d] = __uint_as_float(unordered_key(hi, true));
+  }
+  __syncthreads();
 }
 
-__device__ inline uint32_t morton_of(const float *q, const uint32_t *acc, int D)
+__device__ inline uint32_t morton_of(const float *q, const float *lo_, const float *hi_, int D)
 {
   const int b = MORTON_BITS / D > 0 ? MORTON_BITS / D : 1;
   uint32_t code = 0;
   for (int d = 0; d < D && d < MORTON_BITS; d++) {
-    const float lo = __uint_as_float(unordered_key(acc[d], true));
-    const float hi = __uint_as_float(unordered_key(acc[D + d], true));
+    const float lo = lo_[d];
+    const float hi = hi_[d];
     const float w = hi - lo;
     float t = w > 0.f ? (q[d] - lo) / w : 0.f;
     int c = (int)(t * (float)(1 << b));
@@ -61,32 +101,33 @@ __device__ inline uint32_t morton_of(const float *q, const uint32_t *acc, int D)
   return code & ((1u << MORTON_BITS) - 1);
 }
 
-__global__ void k_qhist(uint32_t *hist, const float *q, int nq, int D, const uint32_t *acc)
+// bucket counts; each query keeps its bucket and its rank inside it
+__global__ __launch_bounds__(256) void k_qhist(uint32_t *hist, uint2 *code_rank, const float *q,
+                                              int nq, int D, const uint32_t *acc)
+{
+  __shared__ float s_lo[CUKD_MAX_DIMS], s_hi[CUKD_MAX_DIMS];
+  qbox_load(acc, D, s_lo, s_hi);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const uint32_t c = morton_of(q + (size_t)i * D, s_lo, s_hi, D);
+  code_rank[i] = make_uint2(c, atomicAdd(&hist[c], 1u));
+}
+
+__global__ void k_qscatter(const uint32_t *starts, int32_t *order, const uint2 *code_rank, int nq)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nq) return;
-  atomicAdd(&hist[morton_of(q + (size_t)i * D, acc, D)], 1u);
+  const uint2 cr = code_rank[i];
+  order[starts[cr.x] + cr.y] = i;
 }
 
-__global__ void k_qscatter(uint32_t *starts, int32_t *order, const float *q, int nq, int D,
-                           const uint32_t *acc)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nq) return;
-  const uint32_t pos = atomicAdd(&starts[morton_of(q + (size_t)i * D, acc, D)], 1u);
-  order[pos] = i;
-}
-
-__global__ void k_qzero(uint32_t *p, int n)
-{
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
-}
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
 size_t query_order_temp_bytes(int nq)
 {
   const size_t buckets = (size_t)1 << MORTON_BITS;
-  return ((size_t)nq * 4 + 255) / 256 * 256 + 2 * buckets * 4 + scan_scratch_words((long long)buckets) * 4 +
-         2 * CUKD_MAX_DIMS * 4 + 1024;
+  return align256((size_t)nq * 4) + align256((size_t)nq * 8) + 2 * buckets * 4 +
+         align256(scan_scratch_words((long long)buckets) * 4) + QB_WORDS * 4 + 256;
 }
 
 // Writes the launch order of nq queries into the front of 'tmp' (which
@@ -96,25 +137,31 @@ int32_t *make_query_order(void *tmp, const float *q, int nq, int D, hipStream_t 
   const int buckets = 1 << MORTON_BITS;
   uint8_t *t = (uint8_t *)tmp;
   int32_t *order = (int32_t *)t;
-  t += ((size_t)nq * 4 + 255) / 256 * 256;
+  if (nq <= 0) return order;
+  t += align256((size_t)nq * 4);
+  uint2 *code_rank = (uint2 *)t;
+  t += align256((size_t)nq * 8);
   uint32_t *hist = (uint32_t *)t;
   t += (size_t)buckets * 4;
   uint32_t *starts = (uint32_t *)t;
   t += (size_t)buckets * 4;
-  uint32_t *acc = (uint32_t *)t;
-  t += 2 * CUKD_MAX_DIMS * 4;
   uint32_t *scratch = (uint32_t *)t;
-  hipLaunchKernelGGL(k_qbounds_init, dim3(1), dim3(64), 0, s, acc, D);
-  // few workgroups: each ends with 2*D atomics on the same 2*D words
-  int blocks = (nq + 255) / 256;
-  if (blocks > 128) blocks = 128;
-  hipLaunchKernelGGL(k_qbounds, dim3(blocks), dim3(256), 0, s, acc, q, nq, D);
-  hipLaunchKernelGGL(k_qzero, dim3(256), dim3(256), 0, s, hist, buckets);
+  t += align256(scan_scratch_words((long long)buckets) * 4);
+  uint32_t *acc = (uint32_t *)t;
+  hipLaunchKernelGGL(k_qinit, dim3(256), dim3(256), 0, s, hist, buckets, acc);
+  const long long E = (long long)nq * D;
+  long long blocks = (E + QB_NT * 4 - 1) / (QB_NT * 4);
+  if (blocks > 1024) blocks = 1024;
+  long long T = blocks * QB_NT;
+  T -= T % D;
+  if (T < D) T = D;  // E >= D, and blocks * QB_NT >= 256 >= D
+  hipLaunchKernelGGL(k_qbounds, dim3((unsigned)blocks), dim3(QB_NT), 0, s, acc, q, E, D, T);
   const int g = (nq + 255) / 256;
-  hipLaunchKernelGGL(k_qhist, dim3(g), dim3(256), 0, s, hist, q, nq, D, (const uint32_t *)acc);
-  scan_exclusive(hist, starts, buckets, scratch, s);
-  hipLaunchKernelGGL(k_qscatter, dim3(g), dim3(256), 0, s, starts, order, q, nq, D,
+  hipLaunchKernelGGL(k_qhist, dim3(g), dim3(256), 0, s, hist, code_rank, q, nq, D,
                      (const uint32_t *)acc);
+  scan_exclusive(hist, starts, buckets, scratch, s);
+  hipLaunchKernelGGL(k_qscatter, dim3(g), dim3(256), 0, s, (const uint32_t *)starts, order,
+                     (const uint2 *)code_rank, nq);
   return order;
 }
 

@@ -278,6 +278,169 @@ __global__ __launch_bounds__(256) void k_query(TreeView t, const float *queries,
 }
 
 // ---------------------------------------------------------------------------
+// batch kernel: two stack-based queries per lane
+// ---------------------------------------------------------------------------
+// The stack-based traversal is a chain of dependent node loads, and a
+// wave holds at most one load per lane in flight.  Here each lane steps
+// two queries together: every iteration issues the node loads of both
+// before using either, so twice as many loads are in flight per wave slot
+// for little more than one query's registers.  Each query's steps are
+// those of traverse_stack_based (include/cukd/traverse.h), one node load
+// per step:
+//   descend  visit node curr, mark its far child pending if the plane is
+//            within the cull distance, go to the close child; past the
+//            last node, pop
+//   pop      re-read the parent of the deepest pending far child and enter
+//            that child if its plane is within the cull distance, else pop
+//            the next one
+// so every query visits the same nodes in the same order with the same
+// candidate updates, and its result is bit-identical to one query per lane.
+//
+// The step is written for few vector instructions, since a wave executes
+// every path some lane takes: node byte offsets are 32-bit (trees under
+// 4 GiB) and follow the descent by doubling, the level is counted rather
+// than recomputed, and descend and pop share the plane test.
+enum { SB_DESCEND = 0, SB_POP = 1, SB_DONE = 2 };
+
+// round-robin split dim of level l (0 <= l < 32)
+template <int D>
+__device__ inline int rr_dim(int l)
+{
+  if ((D & (D - 1)) == 0) return l & (D - 1);
+  if (D == 3) return l - (int)__umul24((unsigned)__umul24((unsigned)l, 11u) >> 5, 3u);  // l / 3 for l < 32
+  return l % D;
+}
+
+struct SbSlot {
+  int mode;
+  int curr;          // descend: node to visit; pop: end of the path (>= n)
+  int lvl;           // level of curr
+  int pd;            // pop: depth of the far child being tested
+  unsigned off;      // byte offset of the node loaded next (curr, or the far child's parent)
+  unsigned pending;  // bit d: the far child at depth d of the path is still to be visited
+  float cull;
+};
+
+template <int D, class R>
+__device__ inline void sb_step(R &res, const float (&q)[D], const float (&p)[D], int n,
+                               unsigned stride, SbSlot &s, unsigned &visits)
+{
+  if (s.mode == SB_DONE) return;
+  const bool desc = s.mode == SB_DESCEND;
+  const int dim = rr_dim<D>(desc ? s.lvl : s.pd - 1);
+  const float nc = cukd::detail::pick<D>(p, dim);
+  const float qc = cukd::detail::pick<D>(q, dim);
+  if (desc) {
+    visits++;
+    s.cull = res.processCandidate(s.curr, cukd::detail::sqr_dist<D>(p, q));
+  }
+  const float diff = qc - nc;
+  const bool near = diff * diff < s.cull;
+  bool pop;
+  if (desc) {
+    const bool left_close = qc < nc;
+    const int l = 2 * s.curr + 1;
+    const int far = left_close ? l + 1 : l;
+    if (near && far < n) s.pending |= 2u << s.lvl;
+    s.curr = left_close ? l : l + 1;
+    s.off = 2u * s.off + (left_close ? stride : 2u * stride);
+    s.lvl++;
+    pop = s.curr >= n;
+  } else {
+    pop = !near;
+    if (near) {
+      // the far child: sibling of the path's node at depth pd
+      const int f = (((s.curr + 1) >> (s.lvl - s.pd)) ^ 1) - 1;
+      s.off = 2u * s.off + ((f & 1) ? stride : 2u * stride);
+      s.curr = f;
+      s.lvl = s.pd;
+      s.mode = SB_DESCEND;
+    }
+  }
+  if (pop) {
+    if (!s.pending) {
+      s.mode = SB_DONE;
+      return;
+    }
+    const int d = 31 - __clz((int)s.pending);
+    s.pending &= ~(1u << d);
+    s.pd = d;
+    s.mode = SB_POP;
+    s.off = (unsigned)(((s.curr + 1) >> (s.lvl - d + 1)) - 1) * stride;
+  }
+}
+
+// result types whose per-lane state is registers only (trees with
+// round-robin split dims only)
+template <class R> struct DualLane { enum { ok = 0 }; };
+template <> struct DualLane<cukd::FCPResult> { enum { ok = 1 }; };
+template <int K> struct DualLane<cukd::FixedCandidateList<K>> { enum { ok = 1 }; };
+
+template <int D, class R, bool FOREST>
+__global__ __launch_bounds__(256) void k_query2(TreeView t, const float *queries, int nq,
+                                                float cutoff, int k, const int32_t *order,
+                                                int32_t *ids, float *dist2, float *ret,
+                                                ListArgs la, ForestQ fq,
+                                                unsigned long long *stats)
+{
+  // XCD-aware workgroup order as in k_query; workgroup lb covers queries
+  // [lb*2*nt, (lb+1)*2*nt), wave w of it the 128 from lb*2*nt + 128*w, and
+  // lane l of the wave the two at +l and +64+l
+  const int nb = (int)gridDim.x;
+  const int xcd = (int)blockIdx.x & 7, q8 = nb >> 3, r8 = nb & 7;
+  const int lb = xcd * q8 + (xcd < r8 ? xcd : r8) + ((int)blockIdx.x >> 3);
+  const int i0 = lb * 2 * (int)blockDim.x + ((int)threadIdx.x >> 6) * 128 + ((int)threadIdx.x & 63);
+  if (i0 >= nq) return;
+  const bool has_b = i0 + 64 < nq;
+  const int qa = order ? order[i0] : i0;
+  const int qb = has_b ? (order ? order[i0 + 64] : i0 + 64) : qa;
+  const unsigned stride = (unsigned)t.stride;
+  const uint8_t *base = t.base + t.point_offset;
+  unsigned tb_a = 0, tb_b = 0;  // byte offset of the slot's tree in the forest
+  int n_a = t.n, n_b = t.n;
+  if constexpr (FOREST) {
+    const int ta = fq.seg_tree[segment_of(fq.seg_off, fq.nseg, qa)];
+    const int tb = fq.seg_tree[segment_of(fq.seg_off, fq.nseg, qb)];
+    const int oa = fq.tree_off[ta], ob = fq.tree_off[tb];
+    tb_a = (unsigned)oa * stride;
+    tb_b = (unsigned)ob * stride;
+    n_a = fq.tree_off[ta + 1] - oa;
+    n_b = fq.tree_off[tb + 1] - ob;
+  }
+  float pqa[D], pqb[D];
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    pqa[d] = queries[(size_t)qa * D + d];
+    pqb[d] = queries[(size_t)qb * D + d];
+  }
+  typename ResultOps<R>::State sta, stb;
+  R &ra = ResultOps<R>::init(sta, cutoff, k, qa, la);
+  R &rb = ResultOps<R>::init(stb, cutoff, k, qb, la);
+  SbSlot sa{n_a > 0 ? SB_DESCEND : SB_DONE, 0, 0, 0, 0u, 0u, ra.initialCullDist2()};
+  SbSlot sb{(has_b && n_b > 0) ? SB_DESCEND : SB_DONE, 0, 0, 0, 0u, 0u, rb.initialCullDist2()};
+  unsigned visits = 0;
+  while (sa.mode != SB_DONE || sb.mode != SB_DONE) {
+    // a finished query re-reads the other's node, so both loads stay in
+    // bounds and unconditional
+    unsigned oa = tb_a + sa.off, ob = tb_b + sb.off;
+    if (sa.mode == SB_DONE) oa = ob;
+    if (sb.mode == SB_DONE) ob = oa;
+    const float *src_a = (const float *)(base + oa);
+    const float *src_b = (const float *)(base + ob);
+    float pa[D], pb[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) pa[d] = src_a[d];
+#pragma unroll
+    for (int d = 0; d < D; d++) pb[d] = src_b[d];
+    sb_step<D>(ra, pqa, pa, n_a, stride, sa, visits);
+    sb_step<D>(rb, pqb, pb, n_b, stride, sb, visits);
+  }
+  ResultOps<R>::write(sta, qa, k, ids, dist2, ret, la);
+  if (has_b) ResultOps<R>::write(stb, qb, k, ids, dist2, ret, la);
+  if (stats) atomicAdd(stats, (unsigned long long)visits);
+}
+
+// ---------------------------------------------------------------------------
 // per-dimension launchers
 // ---------------------------------------------------------------------------
 struct QueryArgs {
@@ -297,6 +460,8 @@ struct QueryArgs {
   ForestQ fq;
   unsigned long long *stats;
   int block = 128;  // workgroup size: 64, 128 or 256
+  int lanes = 1;    // queries per lane: 1, or 2 (stack-based, register lists)
+  uint64_t tree_bytes = 0;  // bytes of the tree array (forests: all trees)
 };
 
 // persistent multi-round lists (cukd_knn_accumulate)
@@ -306,6 +471,18 @@ template <int D, int METHOD, class R>
 inline hipError_t launch_q(const QueryArgs &a, hipStream_t s)
 {
   const int nt = a.block;
+  if constexpr (METHOD == CUKD_STACK_BASED && DualLane<R>::ok) {
+    if (a.lanes == 2 && a.t.dim_offset < 0) {
+      const int blocks = (a.nq + 2 * nt - 1) / (2 * nt);
+      if (a.fq.seg_off)
+        hipLaunchKernelGGL((k_query2<D, R, true>), dim3(blocks), dim3(nt), 0, s, a.t, a.queries,
+                           a.nq, a.cutoff, a.k, a.order, a.ids, a.dist2, a.ret, a.la, a.fq, a.stats);
+      else
+        hipLaunchKernelGGL((k_query2<D, R, false>), dim3(blocks), dim3(nt), 0, s, a.t, a.queries,
+                           a.nq, a.cutoff, a.k, a.order, a.ids, a.dist2, a.ret, a.la, a.fq, a.stats);
+      return hipGetLastError();
+    }
+  }
   const int blocks = (a.nq + nt - 1) / nt;
   hipLaunchKernelGGL((k_query<D, METHOD, R>), dim3(blocks), dim3(nt), 0, s, a.t, a.queries,
                      a.nq, a.world_bounds, a.cutoff, a.eps, a.k, a.order, a.ids, a.dist2,

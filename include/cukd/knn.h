@@ -61,9 +61,16 @@ struct FixedCandidateList : public CandidateList<k> {
     bool gt[k];
 #pragma unroll
     for (int i = 0; i < k; i++) gt[i] = entry[i] > v;
+    // The selects take an opaque copy of v: otherwise the compiler turns
+    // "gt[i] ? v : entry[i]" into min(entry[i], v) and compares every slot
+    // twice.
+    uint64_t w = v;
+#ifdef __HIP_DEVICE_COMPILE__
+    asm("" : "+v"(w));
+#endif
 #pragma unroll
-    for (int i = k - 1; i > 0; i--) entry[i] = gt[i - 1] ? entry[i - 1] : (gt[i] ? v : entry[i]);
-    entry[0] = gt[0] ? v : entry[0];
+    for (int i = k - 1; i > 0; i--) entry[i] = gt[i - 1] ? entry[i - 1] : (gt[i] ? w : entry[i]);
+    entry[0] = gt[0] ? w : entry[0];
   }
   inline __both__ float processCandidate(int candPrimID, float candDist2)
   {

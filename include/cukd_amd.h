@@ -82,11 +82,19 @@ typedef struct cukd_mem_resource {
                              (results are returned in input order)      */
 #define CUKD_QUERY_AUTOTUNE 2 /* first call per (dims, traversal, list, k,
                              log2 nq): time workgroup sizes 64/128/256 with
-                             and without Morton ordering on the caller's
+                             and without Morton ordering, and one or two
+                             queries per lane where two apply, on the caller's
                              inputs (synchronises the stream once), keep
                              the fastest for later calls; results are the
                              same for every choice.  Not for multi-round
                              lists (they only use an already tuned choice) */
+#define CUKD_QUERY_ONE_PER_LANE 4 /* one query per lane everywhere.  By
+                             default the stack-based traversal with FCP or
+                             a FixedCandidateList of k = 1/4/8/16 over a
+                             tree with round-robin split dims (and under
+                             4 GiB) steps two queries per lane, so twice as
+                             many node loads are in flight; results are
+                             identical either way */
 
 /* Builds a left-balanced k-d tree in place over d_points[0..n).
  * Replaces template<data_t,traits> buildTree / buildTree_thrust /

@@ -21,6 +21,21 @@ def test_header_and_binding_agree():
     assert _declared() == sorted(ck.EXPORTED_SYMBOLS)
 
 
+def test_python_constants_match_header_defines():
+    txt = open(os.path.join(ROOT, "include", "cukd_amd.h")).read()
+    defs = dict((m.group(1), int(m.group(2)))
+                for m in re.finditer(r"#define CUKD_([A-Z0-9_]+) \(?(-?\d+)\)?", txt))
+    checked = 0
+    for name, val in defs.items():
+        if hasattr(ck, name):
+            assert getattr(ck, name) == val, name
+            checked += 1
+    for name in ("QUERY_SORT", "QUERY_AUTOTUNE", "QUERY_ONE_PER_LANE", "STACK_BASED", "CCT",
+                 "LIST_HEAP", "BUILDER_INPLACE", "ERR_UNSUPPORTED"):
+        assert name in defs and hasattr(ck, name), name
+    assert checked >= 20
+
+
 def test_library_exports_every_symbol():
     out = subprocess.run(["nm", "-D", "--defined-only", ck._LIB_PATH],
                          capture_output=True, text=True, check=True).stdout

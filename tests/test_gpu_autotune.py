@@ -71,3 +71,33 @@ def test_forest_sorted_and_autotuned_queries():
         for a, b in zip(got, base):
             assert torch.equal(a, b), flags
     f.close()
+
+
+
+
+def test_query_order_is_a_morton_permutation():
+    """cukd_query_order returns a permutation whose queries have
+    non-decreasing Morton cells, and launching in that order changes no
+    result."""
+    D = 3
+    q = datagen.gmm(50000, D, 11)
+    dq = torch.from_numpy(q).cuda()
+    order = ck.query_order(dq).cpu().numpy()
+    assert np.array_equal(np.sort(order), np.arange(len(q)))
+    lo, hi = q.min(0), q.max(0)  # float32, as on the device
+    t = (q - lo) / (hi - lo)
+    cell = np.clip((t * np.float32(64)).astype(np.int64), 0, 63)
+    code = np.zeros(len(q), np.int64)
+    for j in range(6):
+        for d in range(D):
+            code |= ((cell[:, d] >> j) & 1) << (j * D + d)
+    assert np.all(np.diff(code[order]) >= 0)
+    lay = ck.float_points(D)
+    tree, _ = oc.build_host(datagen.uniform(20000, D, 12), lay)
+    dt = torch.from_numpy(tree).cuda()
+    a = ck.knn(dt, lay, dq, 4)
+    b = ck.knn(dt, lay, dq[torch.from_numpy(order).cuda().long()].contiguous(), 4)
+    inv = np.empty_like(order)
+    inv[order] = np.arange(len(order))
+    for x, y in zip(a, b):
+        assert np.array_equal(x.cpu().numpy(), y.cpu().numpy()[inv])
