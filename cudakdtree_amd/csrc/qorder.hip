@@ -69,18 +69,24 @@ __global__ __launch_bounds__(QB_NT) void k_qbounds(uint32_t *acc, const float *q
 }
 
 // the batch box, folded from the accumulators (written by the previous
-// kernel, so plain loads see them)
+// kernel, so plain loads see them): one word per thread, folded in LDS
 __device__ inline void qbox_load(const uint32_t *acc, int D, float *s_lo, float *s_hi)
 {
+  __shared__ uint32_t k_lo[CUKD_MAX_DIMS], k_hi[CUKD_MAX_DIMS];
+  if (threadIdx.x < CUKD_MAX_DIMS) {
+    k_lo[threadIdx.x] = 0xffffffffu;
+    k_hi[threadIdx.x] = 0u;
+  }
+  __syncthreads();
+  for (int w = threadIdx.x; w < QB_WORDS; w += blockDim.x) {
+    const int d = w % (2 * CUKD_MAX_DIMS);
+    if (d < D) atomicMin(&k_lo[d], acc[w]);
+    else if (d >= CUKD_MAX_DIMS && d < CUKD_MAX_DIMS + D) atomicMax(&k_hi[d - CUKD_MAX_DIMS], acc[w]);
+  }
+  __syncthreads();
   if ((int)threadIdx.x < D) {
-    const int d = threadIdx.x;
-    uint32_t lo = 0xffffffffu, hi = 0u;
-    for (int s = 0; s < QB_SLOTS; s++) {
-      lo = min(lo, acc[s * 2 * CUKD_MAX_DIMS + d]);
-      hi = max(hi, acc[s * 2 * CUKD_MAX_DIMS + CUKD_MAX_DIMS + d]);
-    }
-    s_lo[d] = __uint_as_float(unordered_key(lo, true));
-    s_hi[d] = __uint_as_float(unordered_key(hi, true));
+    s_lo[threadIdx.x] = __uint_as_float(unordered_key(k_lo[threadIdx.x], true));
+    s_hi[threadIdx.x] = __uint_as_float(unordered_key(k_hi[threadIdx.x], true));
   }
   __syncthreads();
 }
