@@ -410,6 +410,17 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     kernel_ms = e0.elapsed_time(e1) / args.steps
+    # the same launch with one query per lane (the kernel the default
+    # two-per-lane one replaces), for comparison
+    for _ in range(args.warmup):
+        ck.knn(tree, lay, qs, k, out=out, flags=ck.QUERY_ONE_PER_LANE)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        ck.knn(tree, lay, qs, k, out=out, flags=ck.QUERY_ONE_PER_LANE)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    one_lane_ms = e0.elapsed_time(e1) / args.steps
     del qs
     knn_gbs = knn_bytes / (kernel_ms / 1e3) / 1e9
     print("[bench] knn k=%d: %.3f ms/step (events %.3f ms, kernel alone %.3f ms), visits "
@@ -491,6 +502,7 @@ def main():
             "bitonic_tree_equals_default": other_builds["bitonic"][1],
             "knn_k8_queries_per_s": round(qps, 1),
             "knn_k8_queries_per_s_unordered": round(nq / (plain_ms / 1e3), 1),
+            "knn_k8_kernel_ms_one_query_per_lane": round(one_lane_ms, 4),
             "fcp_queries_per_s": round(nq / (fcp_ms / 1e3), 1),
             "batched_vs_plain_speedup": forest["batched_vs_plain_speedup"] if forest else None,
             "p99_ms": serve["serve_p99_ms"] if serve else None,
@@ -501,8 +513,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(knn_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(knn_gbs / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "k_query<3, stack-based, FixedCandidateList<8>> on Morton-ordered "
-                               "queries",
+                     "kernel": "k_query2<3, FixedCandidateList<8>> (stack-based, two queries "
+                               "per lane) on Morton-ordered queries",
                      "algorithmic_bytes_per_launch": knn_bytes,
                      "avg_launch_ms": round(kernel_ms, 4),
                      "step_ms_with_ordering": round(ev_ms, 4)},
