@@ -804,6 +804,270 @@ __global__ __launch_bounds__(NT) void k_finish(
 }
 
 // ---------------------------------------------------------------------------
+// LDS finish by ranks: round-robin dims, D <= RK_DMAX, subtree roots at
+// level L0 >= D-1.
+// ---------------------------------------------------------------------------
+// What k_finish sorts at level l, inside each subtree, is the order of
+// the lexicographic key (c[l%D], c[(l-1)%D], ..., c[(l-D+1)%D], input
+// index).  With L0 >= D-1 the chain of levels above always covers all D
+// dims, and a repeated dim adds nothing, so that key depends on d = l%D
+// only: there are D fixed orders key_0 .. key_{D-1}.  This kernel builds
+// them once by stable LSD radix passes:
+//   key_{D-1} = sort by idx, then stably by c[0], c[1], ..., c[D-1]
+//   key_d     = key_{d-1} stably sorted by c[d]   (d = 0 .. D-2)
+// (a stable sort by c[d] of (c[d-1], ..., c[d-D]=c[d], idx) order gives
+// (c[d], c[d-1], ..., c[d-D+1], idx)).  Each level then takes the pivot of
+// every subtree from the order of its dim and stably partitions the other
+// orders by child, so the orders stay sorted inside every subtree.  Nodes,
+// pivots and tie-breaks equal k_finish's exactly.
+enum { RK_DMAX = 4 };
+
+template <int CAP, int NT>
+struct RankLDS {
+  uint16_t ord[RK_DMAX][CAP];  // ord[d]: slots in key_d order inside each subtree
+  uint16_t tmp[CAP];
+  uint32_t kbuf[CAP];          // radix key of each slot
+  uint32_t cnt[256 * (NT / 64)];  // radix: [digit][wave] counts; partition: subtree prefixes
+  uint32_t wsum[NT / 64];
+  uint8_t side[CAP];           // per slot: 0 left, 1 right, 2 pivot
+};
+
+// exclusive scan of one value per thread; returns the thread's prefix and
+// leaves the total in *total
+template <int NT>
+__device__ inline uint32_t rk_block_scan(uint32_t v, uint32_t *wsum, uint32_t *total)
+{
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = (uint32_t)__shfl_up((int)incl, o);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (int i = 0; i < NT / 64; i++) {
+    const uint32_t x = wsum[i];
+    before += i < w ? x : 0u;
+    all += x;
+  }
+  __syncthreads();
+  if (total) *total = all;
+  return before + incl - v;
+}
+
+// One stable LSD pass: dst = src reordered by digit (kbuf[slot] >> sh) & 255.
+// Positions >= m hold no element.  Wave w owns positions
+// [w*64*E, (w+1)*64*E), visited in (e, lane) order.
+template <int CAP, int NT>
+__device__ inline void rk_radix_pass(RankLDS<CAP, NT> &S, const uint16_t *src, uint16_t *dst,
+                                     int m, int sh)
+{
+  enum { E = CAP / NT, NW = NT / 64 };
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = lane; i < 256; i += 64) S.cnt[i * NW + w] = 0u;
+  __syncthreads();
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint16_t slot[E];
+  uint32_t dig[E], rank[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int pos = w * 64 * E + e * 64 + lane;
+    const bool valid = pos < m;
+    const uint16_t sl = valid ? src[pos] : (uint16_t)0;
+    const uint32_t d = valid ? (S.kbuf[sl] >> sh) & 255u : 0u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const uint64_t bb = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    const uint32_t before = valid ? S.cnt[d * NW + w] : 0u;
+    rank[e] = before + (uint32_t)__popcll(peers & below);
+    // every peer has read the count before its lowest lane bumps it (the
+    // wave's LDS accesses run in order; this keeps the compiler's order)
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (peers & below) == 0ull) S.cnt[d * NW + w] = before + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+    slot[e] = sl;
+    dig[e] = valid ? d : 256u;
+  }
+  __syncthreads();
+  // exclusive scan of the counts in (digit, wave) order: 4 per thread
+  uint32_t c4[4], sum = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    c4[i] = S.cnt[4 * threadIdx.x + i];
+    sum += c4[i];
+  }
+  uint32_t run = rk_block_scan<NT>(sum, S.wsum, nullptr);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    S.cnt[4 * threadIdx.x + i] = run;
+    run += c4[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; e++)
+    if (dig[e] < 256u) dst[S.cnt[dig[e] * NW + w] + rank[e]] = slot[e];
+  __syncthreads();
+}
+
+// Four passes: arr is stably sorted by the 32-bit kbuf key (tmp is scratch).
+template <int CAP, int NT>
+__device__ inline void rk_sort32(RankLDS<CAP, NT> &S, uint16_t *arr, int m)
+{
+  rk_radix_pass(S, arr, S.tmp, m, 0);
+  rk_radix_pass(S, S.tmp, arr, m, 8);
+  rk_radix_pass(S, arr, S.tmp, m, 16);
+  rk_radix_pass(S, S.tmp, arr, m, 24);
+}
+
+template <int CAP, int NT>
+__device__ inline void rk_load_keys(RankLDS<CAP, NT> &S, const uint8_t *sdata, const uint32_t *sidx,
+                                    int m, int d, const cukd_layout &L, bool isf)
+{
+  for (int i = threadIdx.x; i < m; i += NT)
+    S.kbuf[i] = d < 0 ? sidx[i] : ordered_key(load_coord_bits(sdata + (size_t)i * L.stride, L, d), isf);
+  __syncthreads();
+}
+
+template <int CAP, int NT>
+__global__ __launch_bounds__(NT) void k_finish_rank(
+    TreeSet<FinTree> set, const uint8_t *dataA, const uint8_t *dataB, const uint32_t *idxA,
+    const uint32_t *idxB, uint8_t *out, cukd_layout L)
+{
+  enum { E = CAP / NT };
+  __shared__ RankLDS<CAP, NT> S;
+  __shared__ FinTree s_tree;
+  if (threadIdx.x == 0)
+    s_tree = find_tree(set, (int)blockIdx.x, [](const FinTree &x) { return x.blk0; });
+  __syncthreads();
+  const FinTree tr = s_tree;
+  const int n = tr.n, L0 = tr.L0;
+  const uint8_t *data = ((L0 & 1) ? dataB : dataA) + (size_t)tr.base * L.stride;
+  const uint32_t *idx = ((L0 & 1) ? idxB : idxA) + tr.base;
+  out += (size_t)tr.base * L.stride;
+
+  const bool isf = L.scalar_type == CUKD_SCALAR_F32;
+  const int D = L.num_dims;
+  const int h = num_levels_for(n);
+  const int hs = h - L0;
+  const int Cs = 1 << hs;
+  const int s = (int)blockIdx.x - tr.blk0;
+  LevelGeom g;
+  g.init(n, L0);
+  const int g_node = (1 << L0) - 1 + s;
+  const int m = g.size(s);
+  const int base = L0 > 0 ? g.offset(s) : 0;
+  const uint8_t *sdata = data + (size_t)base * L.stride;
+  const uint32_t *sidx = idx + base;
+
+  // key_{D-1}: identity, then stably by idx, c[0], ..., c[D-1]
+  uint16_t *W = S.ord[D - 1];
+  for (int i = threadIdx.x; i < m; i += NT) W[i] = (uint16_t)i;
+  rk_load_keys(S, sdata, sidx, m, -1, L, isf);
+  rk_sort32(S, W, m);
+  for (int d = 0; d < D; d++) {
+    rk_load_keys(S, sdata, sidx, m, d, L, isf);
+    rk_sort32(S, W, m);
+  }
+  // key_d = key_{d-1} stably sorted by c[d]
+  for (int d = 0; d + 1 < D; d++) {
+    const uint16_t *prev = S.ord[d == 0 ? D - 1 : d - 1];
+    for (int i = threadIdx.x; i < m; i += NT) S.ord[d][i] = prev[i];
+    rk_load_keys(S, sdata, sidx, m, d, L, isf);
+    rk_sort32(S, S.ord[d], m);
+  }
+
+  const int p0 = threadIdx.x * E;
+  for (int j = 0; j < hs; j++) {
+    const int lgB = hs - j;
+    const int B = 1 << lgB;
+    const int first_node = (g_node + 1) * (1 << j) - 1;
+    const int dj = (L0 + j) % D;
+    // pivots and sides from the order of this level's dim
+    {
+      uint16_t *A = S.ord[dj];
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        const int p = p0 + e;
+        if (p >= Cs) continue;
+        const int b = p >> lgB, r = p & (B - 1);
+        const int node = first_node + b;
+        const int sz = subtree_size(node, n);
+        if (r >= sz) continue;
+        const int left = subtree_size(2 * node + 1, n);
+        const uint16_t sl = A[p];
+        if (r == left) {
+          S.side[sl] = 2;
+          copy_elem(out + (size_t)node * L.stride, sdata + (size_t)sl * L.stride, L, dj);
+        } else {
+          S.side[sl] = r < left ? 0 : 1;
+          S.tmp[r < left ? p : (b << lgB) + (B >> 1) + (r - left - 1)] = sl;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < E; e++)
+        if (p0 + e < Cs) A[p0 + e] = S.tmp[p0 + e];
+      __syncthreads();
+    }
+    if (B <= 2) continue;  // the children are leaves or empty: no later level reads the orders
+    // stable partition of the other orders by side, inside each subtree
+    for (int d = 0; d < D; d++) {
+      if (d == dj) continue;
+      uint16_t *A = S.ord[d];
+      uint16_t sl[E];
+      uint8_t sd[E];
+      uint32_t cntL = 0, cntR = 0;
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        const int p = p0 + e;
+        sd[e] = 3;
+        sl[e] = 0;
+        if (p < Cs) {
+          const int b = p >> lgB, r = p & (B - 1);
+          if (r < subtree_size(first_node + b, n)) {
+            sl[e] = A[p];
+            sd[e] = S.side[sl[e]];
+          }
+        }
+        cntL += sd[e] == 0;
+        cntR += sd[e] == 1;
+      }
+      // packed (right << 16 | left) prefix over all positions
+      uint32_t pre = rk_block_scan<NT>((cntR << 16) | cntL, S.wsum, nullptr);
+      uint32_t at[E];
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        at[e] = pre;
+        pre += (sd[e] == 0) ? 1u : ((sd[e] == 1) ? (1u << 16) : 0u);
+        const int p = p0 + e;
+        if (p < Cs && (p & (B - 1)) == 0) S.cnt[p >> lgB] = at[e];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        if (sd[e] > 1) continue;
+        const int p = p0 + e;
+        const int b = p >> lgB;
+        const uint32_t b0 = S.cnt[b];
+        const int np = sd[e] == 0 ? (b << lgB) + (int)((at[e] & 0xffffu) - (b0 & 0xffffu))
+                                  : (b << lgB) + (B >> 1) + (int)((at[e] >> 16) - (b0 >> 16));
+        S.tmp[np] = sl[e];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < E; e++)
+        if (p0 + e < Cs) A[p0 + e] = S.tmp[p0 + e];
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
 
@@ -845,6 +1109,16 @@ hipError_t launch_finish(bool expl, const TreeSet<FinTree> &set, int blocks, hip
   else
     hipLaunchKernelGGL((k_finish<CAP, NT, false>), dim3(blocks), dim3(NT), 0, s, set, dA, dB, iA,
                        iB, st, out, L);
+  return hipGetLastError();
+}
+
+template <int CAP, int NT>
+hipError_t launch_finish_rank(const TreeSet<FinTree> &set, int blocks, hipStream_t s,
+                              const uint8_t *dA, const uint8_t *dB, const uint32_t *iA,
+                              const uint32_t *iB, uint8_t *out, const cukd_layout &L)
+{
+  hipLaunchKernelGGL((k_finish_rank<CAP, NT>), dim3(blocks), dim3(NT), 0, s, set, dA, dB, iA, iB,
+                     out, L);
   return hipGetLastError();
 }
 
@@ -1078,6 +1352,19 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
     if (P.fin_blocks[c] == 0) continue;
     const TreeSet<FinTree> fs = make_set(P, ws, P.tables ? P.off_tab_fin[c] : 0, P.fin[c]);
     const int b = P.fin_blocks[c];
+    // round-robin trees whose finish subtrees see all D dims above them
+    int min_L0 = 1 << 30;
+    for (const FinTree &f : P.fin[c]) min_L0 = f.L0 < min_L0 ? f.L0 : min_L0;
+    if (!expl && D <= RK_DMAX && min_L0 >= D - 1) {
+      switch (c) {
+        case 0: e = launch_finish_rank<512, 256>(fs, b, s, dataA, dataB, idxA, idxB, pts, L); break;
+        case 1: e = launch_finish_rank<1024, 256>(fs, b, s, dataA, dataB, idxA, idxB, pts, L); break;
+        case 2: e = launch_finish_rank<2048, 512>(fs, b, s, dataA, dataB, idxA, idxB, pts, L); break;
+        case 3: e = launch_finish_rank<4096, 512>(fs, b, s, dataA, dataB, idxA, idxB, pts, L); break;
+        default: e = launch_finish_rank<8192, 1024>(fs, b, s, dataA, dataB, idxA, idxB, pts, L); break;
+      }
+      continue;
+    }
     switch (c) {
       case 0: e = launch_finish<512, 256>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
       case 1: e = launch_finish<1024, 256>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
