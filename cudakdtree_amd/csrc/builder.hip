@@ -820,22 +820,24 @@ __global__ __launch_bounds__(NT) void k_finish(
 // every subtree from the order of its dim and stably partitions the other
 // orders by child, so the orders stay sorted inside every subtree.  Nodes,
 // pivots and tie-breaks equal k_finish's exactly.
-enum { RK_DMAX = 4 };
+// RK_CSTRIDE: one wave's 256 digit counters plus one word, so that the
+// lanes of a wave (random digits) and of the scan (same digit, successive
+// waves) spread over the LDS banks
+enum { RK_DMAX = 4, RK_CSTRIDE = 257, RK_RUNMAX = 16 };
 
-template <int CAP, int NT>
+template <int CAP, int NT, int DM>
 struct RankLDS {
-  uint16_t ord[RK_DMAX][CAP];  // ord[d]: slots in key_d order inside each subtree
-  uint16_t tmp[CAP];
-  uint32_t kbuf[CAP];          // radix key of each slot
-  uint32_t cnt[256 * (NT / 64)];  // radix: [digit][wave] counts; partition: subtree prefixes
+  uint16_t buf[DM + 1][CAP];       // the D <= DM orders plus one scratch, by a buffer map
+  uint32_t kbuf[CAP];              // radix key of each slot
+  uint32_t cnt[RK_CSTRIDE * (NT / 64)];  // radix: counts at [wave*RK_CSTRIDE + digit]; partition: subtree prefixes
   uint32_t wsum[NT / 64];
-  uint8_t side[CAP];           // per slot: 0 left, 1 right, 2 pivot
+  uint16_t pivot[CAP];             // slot of each subtree node, by local BFS index
+  uint8_t side[CAP];               // per slot: 0 left, 1 right, 2 pivot
 };
 
-// exclusive scan of one value per thread; returns the thread's prefix and
-// leaves the total in *total
+// exclusive scan of one value per thread; returns the thread's prefix
 template <int NT>
-__device__ inline uint32_t rk_block_scan(uint32_t v, uint32_t *wsum, uint32_t *total)
+__device__ inline uint32_t rk_block_scan(uint32_t v, uint32_t *wsum)
 {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint32_t incl = v;
@@ -846,100 +848,120 @@ __device__ inline uint32_t rk_block_scan(uint32_t v, uint32_t *wsum, uint32_t *t
   }
   if (lane == 63) wsum[w] = incl;
   __syncthreads();
-  uint32_t before = 0, all = 0;
-  for (int i = 0; i < NT / 64; i++) {
-    const uint32_t x = wsum[i];
-    before += i < w ? x : 0u;
-    all += x;
-  }
+  uint32_t before = 0;
+  for (int i = 0; i < NT / 64; i++) before += i < w ? wsum[i] : 0u;
   __syncthreads();
-  if (total) *total = all;
   return before + incl - v;
 }
 
 // One stable LSD pass: dst = src reordered by digit (kbuf[slot] >> sh) & 255.
 // Positions >= m hold no element.  Wave w owns positions
 // [w*64*E, (w+1)*64*E), visited in (e, lane) order.
-template <int CAP, int NT>
-__device__ inline void rk_radix_pass(RankLDS<CAP, NT> &S, const uint16_t *src, uint16_t *dst,
+template <int CAP, int NT, int DM>
+__device__ inline void rk_radix_pass(RankLDS<CAP, NT, DM> &S, const uint16_t *src, uint16_t *dst,
                                      int m, int sh)
 {
   enum { E = CAP / NT, NW = NT / 64 };
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int i = lane; i < 256; i += 64) S.cnt[i * NW + w] = 0u;
-  __syncthreads();
-  const uint64_t below = (1ull << lane) - 1ull;
+  for (int i = lane; i < 256; i += 64) S.cnt[w * RK_CSTRIDE + i] = 0u;
+  // slots and digits first: these loads do not wait on the counters
   uint16_t slot[E];
   uint32_t dig[E], rank[E];
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int pos = w * 64 * E + e * 64 + lane;
-    const bool valid = pos < m;
-    const uint16_t sl = valid ? src[pos] : (uint16_t)0;
-    const uint32_t d = valid ? (S.kbuf[sl] >> sh) & 255u : 0u;
+    slot[e] = pos < m ? src[pos] : (uint16_t)0;
+  }
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int pos = w * 64 * E + e * 64 + lane;
+    dig[e] = pos < m ? (S.kbuf[slot[e]] >> sh) & 255u : 256u;
+  }
+  __syncthreads();
+  const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const bool valid = dig[e] < 256u;
+    const uint32_t d = dig[e] & 255u;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; b++) {
       const uint64_t bb = __ballot((d >> b) & 1u);
       peers &= ((d >> b) & 1u) ? bb : ~bb;
     }
-    const uint32_t before = valid ? S.cnt[d * NW + w] : 0u;
+    const uint32_t before = valid ? S.cnt[w * RK_CSTRIDE + d] : 0u;
     rank[e] = before + (uint32_t)__popcll(peers & below);
     // every peer has read the count before its lowest lane bumps it (the
     // wave's LDS accesses run in order; this keeps the compiler's order)
     __builtin_amdgcn_wave_barrier();
-    if (valid && (peers & below) == 0ull) S.cnt[d * NW + w] = before + (uint32_t)__popcll(peers);
+    if (valid && (peers & below) == 0ull) S.cnt[w * RK_CSTRIDE + d] = before + (uint32_t)__popcll(peers);
     __builtin_amdgcn_wave_barrier();
-    slot[e] = sl;
-    dig[e] = valid ? d : 256u;
   }
   __syncthreads();
   // exclusive scan of the counts in (digit, wave) order: 4 per thread
   uint32_t c4[4], sum = 0;
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    c4[i] = S.cnt[4 * threadIdx.x + i];
+    const int k = 4 * threadIdx.x + i;  // = digit * NW + wave
+    c4[i] = S.cnt[(k % NW) * RK_CSTRIDE + k / NW];
     sum += c4[i];
   }
-  uint32_t run = rk_block_scan<NT>(sum, S.wsum, nullptr);
+  uint32_t run = rk_block_scan<NT>(sum, S.wsum);
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    S.cnt[4 * threadIdx.x + i] = run;
+    const int k = 4 * threadIdx.x + i;
+    S.cnt[(k % NW) * RK_CSTRIDE + k / NW] = run;
     run += c4[i];
   }
   __syncthreads();
 #pragma unroll
   for (int e = 0; e < E; e++)
-    if (dig[e] < 256u) dst[S.cnt[dig[e] * NW + w] + rank[e]] = slot[e];
+    if (dig[e] < 256u) dst[S.cnt[w * RK_CSTRIDE + dig[e]] + rank[e]] = slot[e];
   __syncthreads();
 }
 
-// Four passes: arr is stably sorted by the 32-bit kbuf key (tmp is scratch).
-template <int CAP, int NT>
-__device__ inline void rk_sort32(RankLDS<CAP, NT> &S, uint16_t *arr, int m)
-{
-  rk_radix_pass(S, arr, S.tmp, m, 0);
-  rk_radix_pass(S, S.tmp, arr, m, 8);
-  rk_radix_pass(S, arr, S.tmp, m, 16);
-  rk_radix_pass(S, S.tmp, arr, m, 24);
-}
-
-template <int CAP, int NT>
-__device__ inline void rk_load_keys(RankLDS<CAP, NT> &S, const uint8_t *sdata, const uint32_t *sidx,
+// kbuf[slot] = the slot's input index (d < 0) or ordered coordinate d
+template <int CAP, int NT, int DM>
+__device__ inline void rk_load_keys(RankLDS<CAP, NT, DM> &S, const uint8_t *sdata, const uint32_t *sidx,
                                     int m, int d, const cukd_layout &L, bool isf)
 {
-  for (int i = threadIdx.x; i < m; i += NT)
-    S.kbuf[i] = d < 0 ? sidx[i] : ordered_key(load_coord_bits(sdata + (size_t)i * L.stride, L, d), isf);
+  enum { E = CAP / NT };
+  uint32_t v[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int i = e * NT + threadIdx.x;
+    v[e] = i < m ? (d < 0 ? sidx[i] : load_coord_bits(sdata + (size_t)i * L.stride, L, d)) : 0u;
+  }
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int i = e * NT + threadIdx.x;
+    if (i < m) S.kbuf[i] = d < 0 ? v[e] : ordered_key(v[e], isf);
+  }
   __syncthreads();
 }
 
-template <int CAP, int NT>
+// the buffer holding order d (4 bits per order in 'map')
+__device__ inline int rk_buf(uint32_t map, int d) { return (int)((map >> (4 * d)) & 15u); }
+
+// Stable LSD sort of buffer 'a' by the kbuf key, using buffer 't' as
+// scratch, skipping digit passes at or above 'bits'.  Returns the buffer
+// that holds the result (a or t) and leaves the other as scratch.
+template <int CAP, int NT, int DM>
+__device__ inline void rk_sort(RankLDS<CAP, NT, DM> &S, int &a, int &t, int m, int bits)
+{
+  for (int sh = 0; sh < bits; sh += 8) {
+    rk_radix_pass(S, S.buf[a], S.buf[t], m, sh);
+    const int x = a; a = t; t = x;
+  }
+}
+
+template <int CAP, int NT, int DM>
 __global__ __launch_bounds__(NT) void k_finish_rank(
     TreeSet<FinTree> set, const uint8_t *dataA, const uint8_t *dataB, const uint32_t *idxA,
     const uint32_t *idxB, uint8_t *out, cukd_layout L)
 {
   enum { E = CAP / NT };
-  __shared__ RankLDS<CAP, NT> S;
+  __shared__ RankLDS<CAP, NT, DM> S;
   __shared__ FinTree s_tree;
   if (threadIdx.x == 0)
     s_tree = find_tree(set, (int)blockIdx.x, [](const FinTree &x) { return x.blk0; });
@@ -963,62 +985,141 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
   const int base = L0 > 0 ? g.offset(s) : 0;
   const uint8_t *sdata = data + (size_t)base * L.stride;
   const uint32_t *sidx = idx + base;
+  // input indices are < n: higher idx digits are zero
+  const int idx_bits = n > 1 ? 32 - __clz(n - 1) : 1;
 
-  // key_{D-1}: identity, then stably by idx, c[0], ..., c[D-1]
-  uint16_t *W = S.ord[D - 1];
-  for (int i = threadIdx.x; i < m; i += NT) W[i] = (uint16_t)i;
-  rk_load_keys(S, sdata, sidx, m, -1, L, isf);
-  rk_sort32(S, W, m);
+  // Fast path: order each dim by its coordinate alone (four passes), then
+  // reorder every run of equal coordinates by the rest of its key.  Runs
+  // longer than RK_RUNMAX send the subtree to the exact LSD chain below.
+  __shared__ int s_long;
+  if (threadIdx.x == 0) s_long = 0;
+  uint32_t map = 0;
+  int t = DM;
   for (int d = 0; d < D; d++) {
+    int a = d;
+    for (int i = threadIdx.x; i < m; i += NT) S.buf[a][i] = (uint16_t)i;
     rk_load_keys(S, sdata, sidx, m, d, L, isf);
-    rk_sort32(S, W, m);
+    rk_sort(S, a, t, m, 32);  // four passes: the result is back in buffer d
+    uint16_t *A = S.buf[a];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int q = threadIdx.x * E + e;
+      if (q + 1 >= m) continue;
+      const uint32_t kq = S.kbuf[A[q]];
+      if (q > 0 && S.kbuf[A[q - 1]] == kq) continue;  // not a run start
+      int len = 1;
+      while (q + len < m && len <= RK_RUNMAX && S.kbuf[A[q + len]] == kq) len++;
+      if (len == 1) continue;
+      if (len > RK_RUNMAX) {
+        s_long = 1;
+        continue;
+      }
+      // insertion sort of the run by (c[d-1], ..., c[d-D+1], idx)
+      for (int i = 1; i < len; i++) {
+        const uint16_t x = A[q + i];
+        int k = i;
+        while (k > 0) {
+          const uint16_t y = A[q + k - 1];
+          bool less = false, decided = false;
+          for (int c = 1; c < D && !decided; c++) {
+            const int dd = (d - c + D) % D;
+            const uint32_t kx = ordered_key(load_coord_bits(sdata + (size_t)x * L.stride, L, dd), isf);
+            const uint32_t ky = ordered_key(load_coord_bits(sdata + (size_t)y * L.stride, L, dd), isf);
+            if (kx != ky) {
+              less = kx < ky;
+              decided = true;
+            }
+          }
+          if (!decided) less = sidx[x] < sidx[y];
+          if (!less) break;
+          A[q + k] = y;
+          k--;
+        }
+        A[q + k] = x;
+      }
+    }
+    __syncthreads();
+    if (s_long) break;
+    map |= (uint32_t)a << (4 * d);
   }
-  // key_d = key_{d-1} stably sorted by c[d]
-  for (int d = 0; d + 1 < D; d++) {
-    const uint16_t *prev = S.ord[d == 0 ? D - 1 : d - 1];
-    for (int i = threadIdx.x; i < m; i += NT) S.ord[d][i] = prev[i];
-    rk_load_keys(S, sdata, sidx, m, d, L, isf);
-    rk_sort32(S, S.ord[d], m);
+  if (s_long) {
+    // exact chain.  key_{D-1}: identity, then stably by idx, c[0], ..., c[D-1]
+    int a = 0;
+    t = DM;
+    for (int i = threadIdx.x; i < m; i += NT) S.buf[a][i] = (uint16_t)i;
+    rk_load_keys(S, sdata, sidx, m, -1, L, isf);
+    rk_sort(S, a, t, m, idx_bits);
+    for (int d = 0; d < D; d++) {
+      rk_load_keys(S, sdata, sidx, m, d, L, isf);
+      rk_sort(S, a, t, m, 32);
+    }
+    // buffers: order D-1 in a, scratch t, the others free for orders 0..D-2
+    map = 0;
+    {
+      int next = 0;
+      for (int d = 0; d + 1 < D; d++) {
+        while (next == a || next == t) next++;
+        map |= (uint32_t)next << (4 * d);
+        next++;
+      }
+      map |= (uint32_t)a << (4 * (D - 1));
+    }
+    // key_d = key_{d-1} stably sorted by c[d]
+    for (int d = 0; d + 1 < D; d++) {
+      int bd = rk_buf(map, d);
+      const uint16_t *prev = S.buf[rk_buf(map, d == 0 ? D - 1 : d - 1)];
+      for (int i = threadIdx.x; i < m; i += NT) S.buf[bd][i] = prev[i];
+      rk_load_keys(S, sdata, sidx, m, d, L, isf);
+      rk_sort(S, bd, t, m, 32);
+      map = (map & ~(15u << (4 * d))) | ((uint32_t)bd << (4 * d));
+    }
   }
 
   const int p0 = threadIdx.x * E;
+  uint32_t *sizes = S.kbuf;  // per level: subtree size | left-child size << 16, per block
   for (int j = 0; j < hs; j++) {
     const int lgB = hs - j;
     const int B = 1 << lgB;
     const int first_node = (g_node + 1) * (1 << j) - 1;
     const int dj = (L0 + j) % D;
+    for (int b = threadIdx.x; b < (1 << j); b += NT) {
+      const int node = first_node + b;
+      sizes[b] = (uint32_t)subtree_size(node, n) | ((uint32_t)subtree_size(2 * node + 1, n) << 16);
+    }
+    __syncthreads();
     // pivots and sides from the order of this level's dim
     {
-      uint16_t *A = S.ord[dj];
+      const uint16_t *A = S.buf[rk_buf(map, dj)];
+      uint16_t *T = S.buf[t];
 #pragma unroll
       for (int e = 0; e < E; e++) {
         const int p = p0 + e;
         if (p >= Cs) continue;
         const int b = p >> lgB, r = p & (B - 1);
-        const int node = first_node + b;
-        const int sz = subtree_size(node, n);
+        const uint32_t zs = sizes[b];
+        const int sz = (int)(zs & 0xffffu);
         if (r >= sz) continue;
-        const int left = subtree_size(2 * node + 1, n);
+        const int left = (int)(zs >> 16);
         const uint16_t sl = A[p];
         if (r == left) {
           S.side[sl] = 2;
-          copy_elem(out + (size_t)node * L.stride, sdata + (size_t)sl * L.stride, L, dj);
+          S.pivot[(1 << j) - 1 + b] = sl;
         } else {
           S.side[sl] = r < left ? 0 : 1;
-          S.tmp[r < left ? p : (b << lgB) + (B >> 1) + (r - left - 1)] = sl;
+          T[r < left ? p : (b << lgB) + (B >> 1) + (r - left - 1)] = sl;
         }
       }
       __syncthreads();
-#pragma unroll
-      for (int e = 0; e < E; e++)
-        if (p0 + e < Cs) A[p0 + e] = S.tmp[p0 + e];
-      __syncthreads();
+      const int x = rk_buf(map, dj);
+      map = (map & ~(15u << (4 * dj))) | ((uint32_t)t << (4 * dj));
+      t = x;
     }
     if (B <= 2) continue;  // the children are leaves or empty: no later level reads the orders
     // stable partition of the other orders by side, inside each subtree
     for (int d = 0; d < D; d++) {
       if (d == dj) continue;
-      uint16_t *A = S.ord[d];
+      const uint16_t *A = S.buf[rk_buf(map, d)];
+      uint16_t *T = S.buf[t];
       uint16_t sl[E];
       uint8_t sd[E];
       uint32_t cntL = 0, cntR = 0;
@@ -1029,16 +1130,17 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
         sl[e] = 0;
         if (p < Cs) {
           const int b = p >> lgB, r = p & (B - 1);
-          if (r < subtree_size(first_node + b, n)) {
-            sl[e] = A[p];
-            sd[e] = S.side[sl[e]];
-          }
+          if (r < (int)(sizes[b] & 0xffffu)) sl[e] = A[p], sd[e] = 0;
         }
+      }
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        if (sd[e] == 0) sd[e] = S.side[sl[e]];
         cntL += sd[e] == 0;
         cntR += sd[e] == 1;
       }
       // packed (right << 16 | left) prefix over all positions
-      uint32_t pre = rk_block_scan<NT>((cntR << 16) | cntL, S.wsum, nullptr);
+      uint32_t pre = rk_block_scan<NT>((cntR << 16) | cntL, S.wsum);
       uint32_t at[E];
 #pragma unroll
       for (int e = 0; e < E; e++) {
@@ -1056,14 +1158,23 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
         const uint32_t b0 = S.cnt[b];
         const int np = sd[e] == 0 ? (b << lgB) + (int)((at[e] & 0xffffu) - (b0 & 0xffffu))
                                   : (b << lgB) + (B >> 1) + (int)((at[e] >> 16) - (b0 >> 16));
-        S.tmp[np] = sl[e];
+        T[np] = sl[e];
       }
       __syncthreads();
-#pragma unroll
-      for (int e = 0; e < E; e++)
-        if (p0 + e < Cs) A[p0 + e] = S.tmp[p0 + e];
-      __syncthreads();
+      const int x = rk_buf(map, d);
+      map = (map & ~(15u << (4 * d))) | ((uint32_t)t << (4 * d));
+      t = x;
     }
+  }
+  // the nodes, level by level: consecutive local indices are consecutive
+  // tree nodes of one level
+  __syncthreads();
+  for (int local = threadIdx.x; local < Cs - 1; local += NT) {
+    const int j = 31 - __clz(local + 1);
+    const int node = (g_node + 1) * (1 << j) - 1 + (local - ((1 << j) - 1));
+    if (node < n)
+      copy_elem(out + (size_t)node * L.stride, sdata + (size_t)S.pivot[local] * L.stride, L,
+                (L0 + j) % D);
   }
 }
 
@@ -1117,8 +1228,12 @@ hipError_t launch_finish_rank(const TreeSet<FinTree> &set, int blocks, hipStream
                               const uint8_t *dA, const uint8_t *dB, const uint32_t *iA,
                               const uint32_t *iB, uint8_t *out, const cukd_layout &L)
 {
-  hipLaunchKernelGGL((k_finish_rank<CAP, NT>), dim3(blocks), dim3(NT), 0, s, set, dA, dB, iA, iB,
-                     out, L);
+  if (L.num_dims <= 3)
+    hipLaunchKernelGGL((k_finish_rank<CAP, NT, 3>), dim3(blocks), dim3(NT), 0, s, set, dA, dB,
+                       iA, iB, out, L);
+  else
+    hipLaunchKernelGGL((k_finish_rank<CAP, NT, RK_DMAX>), dim3(blocks), dim3(NT), 0, s, set, dA,
+                       dB, iA, iB, out, L);
   return hipGetLastError();
 }
 

@@ -49,7 +49,7 @@ def test_float3_uniform_sizes(n):
 
 @pytest.mark.parametrize("builder", EXACT_BUILDERS)
 @pytest.mark.parametrize("dist", ["uniform", "ties", "dups", "plane", "gmm", "signed_zero",
-                                  "unique"])
+                                  "unique", "coarse"])
 def test_float3_distributions(builder, dist):
     compare(distribution(dist, 30011, 3, seed=2), "float", builder)
 
@@ -87,8 +87,18 @@ def test_builders_same_result_matrix(D, kind, n):
 
 
 def test_two_million_uniform_and_gmm():
-    for dist in ("uniform", "gmm"):
+    for dist in ("uniform", "gmm", "coarse"):
         compare(distribution(dist, 2_000_000, 3, seed=77), "float")
+
+
+@pytest.mark.parametrize("D", [1, 2, 4])
+@pytest.mark.parametrize("kind", ["float", "int", "payload"])
+def test_short_tie_runs(D, kind):
+    """Short runs of equal coordinates inside the LDS-finished subtrees are
+    ordered by the rest of the key (the bitonic and rank finishes must
+    agree with the host builder)."""
+    scalar_float = kind != "int"
+    compare(distribution("coarse", 300_000, D, seed=5 + D, scalar_float=scalar_float), kind)
 
 
 def test_repeat_builds_identical():
