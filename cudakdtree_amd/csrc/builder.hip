@@ -41,6 +41,7 @@ enum { SEL_NT = 256, SEL_ITEMS = 8, SEL_BATCH = 8, DIGIT_MAX_BINS = 2048 };
 // per subtree: DIGIT_MAX_BINS counters, then one bit per bin that some tile
 // touched in the current pass
 enum { HIST_STRIDE = DIGIT_MAX_BINS + DIGIT_MAX_BINS / 32 };
+enum { SEL_HCOPIES = 8 };  // most histogram copies per subtree
 enum { MAX_WORDS = CUKD_MAX_DIMS + 1 };
 
 // Per-subtree selection state of one level.
@@ -315,7 +316,7 @@ __device__ inline int tiles_of(const LevelGeom &g, int s, int T)
 __global__ __launch_bounds__(SEL_NT) void k_select_pass(
     TreeSet<LvlTree> set, int level, int word, int shift, int bits,
     const uint32_t *key, const uint8_t *data, const uint32_t *idx,
-    SegState *st, uint32_t *hist, uint32_t *need, cukd_layout L, int explicit_dim)
+    SegState *st, uint32_t *hist, uint32_t *need, cukd_layout L, int explicit_dim, int hcopies)
 {
   if (word > 0 && __hip_atomic_load(&need[word], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
     return;
@@ -326,7 +327,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
   __shared__ uint32_t s_sink;
   __shared__ uint32_t s_part[SEL_NT];
   __shared__ uint32_t s_pick[3];
-  __shared__ uint32_t s_bm[DIGIT_MAX_BINS / 32];
+  __shared__ uint32_t s_bm[SEL_HCOPIES][DIGIT_MAX_BINS / 32];
   __shared__ LvlTree s_tree;
   if (threadIdx.x == 0)
     s_tree = find_tree(set, (int)blockIdx.x, [](const LvlTree &x) { return x.stile0; });
@@ -340,7 +341,10 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
   idx += tr.base;
   SegState *child_st = st + tr.cseg;
   st += tr.seg;
-  hist += (size_t)tr.hseg * HIST_STRIDE;
+  // each subtree owns hcopies histograms; a tile flushes into one of them,
+  // so few tiles add into the same counters (levels with few subtrees)
+  hist += (size_t)tr.hseg * hcopies * HIST_STRIDE;
+  const int hcopy = (int)blockIdx.x & (hcopies - 1);
   const bool isf = L.scalar_type == CUKD_SCALAR_F32;
   const int nbins = 1 << bits;
   const int t0 = ((int)blockIdx.x - tr.stile0) * T;
@@ -409,7 +413,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
   uint32_t sink = 0;
   for (int sl = 0; sl < nslots; sl++) {
     const int s = s_seg[sl];
-    uint32_t *h = hist + (size_t)s * HIST_STRIDE;
+    uint32_t *h = hist + ((size_t)s * hcopies + hcopy) * HIST_STRIDE;
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
       const uint32_t c = s_hist[sl][b];
       if (c) sink |= atomicAdd(&h[b], c);
@@ -437,16 +441,23 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
   for (int sl = 0; sl < nslots; sl++) {
     if (!s_last[sl]) continue;
     const int s = s_seg[sl];
-    uint32_t *h = hist + (size_t)s * HIST_STRIDE;
+    uint32_t *h = hist + (size_t)s * hcopies * HIST_STRIDE;
     // reuse s_hist[sl] for the subtree's global histogram.  Read-modify-
     // write atomics are performed where every tile's flush was (plain or
     // relaxed loads may be served by a stale copy in this XCD's L2), and
-    // the bitmap limits them to the bins some tile touched; reading with
+    // the bitmaps limit them to the bins some tile touched; reading with
     // exchanges also clears the counters for the next pass.
-    for (int w = threadIdx.x; w < nbins / 32; w += blockDim.x) s_bm[w] = atomicExch(&h[DIGIT_MAX_BINS + w], 0u);
+    for (int i = threadIdx.x; i < hcopies * (nbins / 32); i += blockDim.x) {
+      const int c = i / (nbins / 32), w = i % (nbins / 32);
+      s_bm[c][w] = atomicExch(&h[(size_t)c * HIST_STRIDE + DIGIT_MAX_BINS + w], 0u);
+    }
     __syncthreads();
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x)
-      s_hist[sl][b] = ((s_bm[b >> 5] >> (b & 31)) & 1u) ? atomicExch(&h[b], 0u) : 0u;
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
+      uint32_t sum = 0;
+      for (int c = 0; c < hcopies; c++)
+        if ((s_bm[c][b >> 5] >> (b & 31)) & 1u) sum += atomicExch(&h[(size_t)c * HIST_STRIDE + b], 0u);
+      s_hist[sl][b] = sum;
+    }
     __syncthreads();
     if (!s_st[sl].resolved && (int)s_st[sl].word == word) {
       // the bin holding rank krem: per-thread sums of consecutive bins, an
@@ -1446,6 +1457,14 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
     hipLaunchKernelGGL(k_level_init, dim3((P.lvl_segs[level] + 255) / 256), dim3(256), 0, s, set,
                        P.lvl_segs[level], st, cursors, need, level, D, expl ? 1 : 0);
     const int max_words = expl ? D + 1 : (level + 1 < D ? level + 1 : D) + 1;
+    // histogram copies per subtree where many tiles add into one subtree
+    // (at least 32 tiles per copy, since the last tile reads every copy), as
+    // many as the buffer (sized for the level with the most subtrees) holds
+    const int tiles_per_seg = P.lvl_stiles[level] / (P.lvl_segs[level] > 0 ? P.lvl_segs[level] : 1);
+    int hcopies = 1;
+    while (hcopies < SEL_HCOPIES && tiles_per_seg >= 64 * hcopies &&
+           (size_t)P.lvl_segs[level] * hcopies * 2 <= P.max_hseg)
+      hcopies *= 2;
     for (int w = 0; w < max_words; w++) {
       const int shifts[3] = {21, 10, 0};
       const int bitsv[3] = {11, 11, 10};
@@ -1453,7 +1472,7 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
         hipLaunchKernelGGL(k_select_pass, dim3(P.lvl_stiles[level]), dim3(SEL_NT), 0, s, set,
                            level, w, shifts[d], bitsv[d], (const uint32_t *)kcur,
                            (const uint8_t *)dcur, (const uint32_t *)icur, st, hist, need, L,
-                           expl ? 1 : 0);
+                           expl ? 1 : 0, hcopies);
     }
     hipLaunchKernelGGL(k_partition, dim3(P.lvl_ptiles[level]), dim3(SEL_NT), 0, s, set, level,
                        (const uint32_t *)kcur, (const uint8_t *)dcur, (const uint32_t *)icur,
