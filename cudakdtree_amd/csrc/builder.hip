@@ -547,6 +547,8 @@ __global__ __launch_bounds__(SEL_NT) void k_partition(
   __shared__ uint32_t s_cdim[4];
   __shared__ uint32_t s_cnt[SEL_ITEMS][SEL_NT / 64][4];
   __shared__ uint32_t s_base[4];
+  __shared__ int s_dst[SEL_ITEMS * SEL_NT];     // per tile element: child slot, or -(node+1) for the pivot
+  __shared__ uint8_t s_edim[SEL_ITEMS * SEL_NT];  // child key dim, or the pivot's split dim
   __shared__ LvlTree s_tree;
   if (threadIdx.x == 0)
     s_tree = find_tree(set, (int)blockIdx.x, [](const LvlTree &x) { return x.ptile0; });
@@ -584,21 +586,31 @@ __global__ __launch_bounds__(SEL_NT) void k_partition(
   __syncthreads();
   // tiles inside one subtree use slot 0 only
   const int seg1_begin = seg1 != seg0 ? g.offset(seg1) : g.n_elems;
-  // classify
+  // classify (all key loads in flight first; the element and index only
+  // for subtrees still tied on a later word)
+  uint32_t kv[SEL_ITEMS];
+#pragma unroll
+  for (int k = 0; k < SEL_ITEMS; k++) {
+    const int p = t0 + k * SEL_NT + threadIdx.x;
+    kv[k] = p < t1 ? key[p] : 0u;
+  }
   uint8_t cls[SEL_ITEMS]; // bit0-1: side (0 left, 1 right, 2 pivot), bit2: slot
+#pragma unroll
   for (int k = 0; k < SEL_ITEMS; k++) {
     const int p = t0 + k * SEL_NT + threadIdx.x;
     cls[k] = 0xff;
     if (p >= t1) continue;
     const int slot = p >= seg1_begin ? 1 : 0;
     const SegState &t = s_st[slot];
-    const uint8_t *elem = data + (size_t)p * L.stride;
-    const uint32_t id = idx[p];
-    const uint32_t k0 = key[p];
-    int side = 2;
-    for (int w = 0; w <= (int)t.word; w++) {
-      const uint32_t v = key_word(w, t, k0, elem, id, L, isf);
-      if (v != t.fixed[w]) { side = v < t.fixed[w] ? 0 : 1; break; }
+    const uint32_t k0 = kv[k];
+    int side = k0 == t.fixed[0] ? 2 : (k0 < t.fixed[0] ? 0 : 1);
+    if (side == 2 && t.word > 0) {
+      const uint8_t *elem = data + (size_t)p * L.stride;
+      const uint32_t id = idx[p];
+      for (int w = 1; w <= (int)t.word; w++) {
+        const uint32_t v = key_word(w, t, k0, elem, id, L, isf);
+        if (v != t.fixed[w]) { side = v < t.fixed[w] ? 0 : 1; break; }
+      }
     }
     cls[k] = (uint8_t)(side | (slot << 2));
   }
@@ -632,26 +644,58 @@ __global__ __launch_bounds__(SEL_NT) void k_partition(
     s_base[cc] = run ? atomicAdd(&cursors[2 * s + (cc & 1)], run) : 0u;
   }
   __syncthreads();
+  // destinations, per tile element
   for (int k = 0; k < SEL_ITEMS; k++) {
+    const int e = k * SEL_NT + (int)threadIdx.x;
     if (cls[k] == 0xff) continue;
-    const int p = t0 + k * SEL_NT + threadIdx.x;
     const int side = cls[k] & 3, slot = cls[k] >> 2;
     const int s = slot ? seg1 : seg0;
-    const uint8_t *elem = data + (size_t)p * L.stride;
     if (side == 2) {
-      const int node = (1 << g.L) - 1 + s;
-      copy_elem(out + (size_t)node * L.stride, elem, L, (int)s_st[slot].dim);
+      s_dst[e] = -((1 << g.L) - 1 + s) - 1;
+      s_edim[e] = (uint8_t)s_st[slot].dim;
       continue;
     }
     const int c = slot * 2 + side;
     const uint32_t rank = s_cnt[k][wave][c] + wrank[k];
-    const int dst = gn.offset(2 * s + side) + (int)(s_base[c] + rank);
-    const int W = L.stride >> 2;
-    uint32_t *dw = (uint32_t *)(ndata + (size_t)dst * L.stride);
-    for (int w = 0; w < W; w++) dw[w] = ((const uint32_t *)elem)[w];
-    nidx[dst] = idx[p];
-    if (write_next_key)
-      nkey[dst] = ordered_key(load_coord_bits(elem, L, (int)s_cdim[c]), isf);
+    s_dst[e] = gn.offset(2 * s + side) + (int)(s_base[c] + rank);
+    s_edim[e] = (uint8_t)s_cdim[c];
+  }
+  __syncthreads();
+  // Copy word by word: the tile's elements are contiguous in the source,
+  // so the loads are coalesced and independent; each element's words go to
+  // its destination (child slot, or the pivot's node).
+  const int W = L.stride >> 2;
+  const int nel = t1 - t0;
+  const int nw = nel * W;
+  const uint32_t *src = (const uint32_t *)(data + (size_t)t0 * L.stride);
+  const uint32_t magic = (uint32_t)((0x100000000ull + (unsigned)W - 1u) / (unsigned)W);  // e = i / W
+  const int cw0 = L.point_offset >> 2;
+  enum { CU = 4 };
+  for (int i0 = (int)threadIdx.x; i0 < nw; i0 += SEL_NT * CU) {
+    uint32_t v[CU];
+#pragma unroll
+    for (int u = 0; u < CU; u++) {
+      const int i = i0 + u * SEL_NT;
+      v[u] = i < nw ? src[i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < CU; u++) {
+      const int i = i0 + u * SEL_NT;
+      if (i >= nw) continue;
+      const int e = W == 1 ? i : (int)__umulhi((uint32_t)i, magic);
+      const int w = i - e * W;
+      const int dst = s_dst[e];
+      if (dst >= 0) {
+        ((uint32_t *)(ndata + (size_t)dst * L.stride))[w] = v[u];
+        if (write_next_key && w == cw0 + (int)s_edim[e]) nkey[dst] = ordered_key(v[u], isf);
+      } else {
+        ((uint32_t *)(out + (size_t)(-dst - 1) * L.stride))[w] = patch_dim_word(v[u], w, L, (int)s_edim[e]);
+      }
+    }
+  }
+  for (int e = (int)threadIdx.x; e < nel; e += SEL_NT) {
+    const int dst = s_dst[e];
+    if (dst >= 0) nidx[dst] = idx[t0 + e];
   }
 }
 
