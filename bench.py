@@ -269,8 +269,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1",
+        print("bench.py measures one GPU: there is no multi-GPU path; run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 

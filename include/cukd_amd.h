@@ -4,7 +4,7 @@
  * entry point names the reference interface it replaces (file:line in the
  * ingowald/cudaKDTree v1.0.1 tree).  The C++ header API under include/cukd/
  * maps the reference's templates onto these calls; other languages bind
- * them directly (see INTEGRATION.md for a ctypes stub).
+ * them directly (cudakdtree_amd/__init__.py is a ctypes binding).
  *
  * Conventions
  *  - All pointers named d_* are device pointers.  Every call is
