@@ -134,11 +134,12 @@ def serve_bench(args, ck, datagen, torch, np):
     in [10K, 2M] (about 190M points), each a 64-centre Gaussian mixture in
     its own random box.  Each request names a tree (uniform) and carries
     1..100K (log-uniform) queries -- data points of that tree plus a small
-    Gaussian jitter -- for FCP or kNN with k in {1, 4, 16}.  The server's
-    saturation rate is measured first (all requests of a burst submitted at
-    once); requests then arrive as a Poisson process at 60% of it, and the
-    same arrival trace is replayed against the batching server and against
-    one launch sequence per request.  Latency = submit -> results in the
+    Gaussian jitter -- for FCP or kNN with k in {1, 4, 16}.  Saturation
+    throughput is measured first, batched and with one launch sequence per
+    request (all requests of a burst submitted at once); requests then
+    arrive as a Poisson process at a fixed rate (--serve-rate), and the same
+    arrival trace is replayed against the batching server and against one
+    launch sequence per request.  Latency = submit -> results in the
     caller's host buffers."""
     dev = torch.device("cuda:0")
     T = args.serve_trees
@@ -187,6 +188,14 @@ def serve_bench(args, ck, datagen, torch, np):
         w = srv.submit(0, reqs[0][1][:16], 4)  # warm-up
         srv.poll(wait_for=1)
         srv.result(w)
+        # result buffers allocated and touched before the clock starts, as a
+        # client reusing its buffers would have them
+        outs = []
+        for i in range(count):
+            n = len(reqs[i][1]) * max(1, reqs[i][2])
+            outs.append((np.zeros(n, np.int32), np.zeros(n, np.float32)))
+            outs[-1][0].fill(1)
+            outs[-1][1].fill(1.0)
         tickets = []
         t0 = time.perf_counter()
         for i in range(count):
@@ -195,7 +204,7 @@ def serve_bench(args, ck, datagen, torch, np):
                 target = t0 + arrivals[i]
                 while time.perf_counter() < target:
                     pass
-            tickets.append(srv.submit(tree, q, k))
+            tickets.append(srv.submit(tree, q, k, out=outs[i]))
         lat = {}
         while len(lat) < count:
             for c in srv.poll(wait_for=1):
@@ -207,14 +216,18 @@ def serve_bench(args, ck, datagen, torch, np):
         srv.close()
         return np.array([lat[t] for t in tickets]), wall, res, int(st.batches)
 
-    # saturation: every request of a burst queued at once
+    # saturation: every request of a burst queued at once, batched and one
+    # launch sequence per request
     nb = min(R, 1000)
     _, burst_wall, _, _ = replay(0, None, nb)
     cap_qps = float(nqs[:nb].sum()) / burst_wall
-    rate = 0.6 * cap_qps / float(nqs.mean())
+    npb = min(R, 300)
+    _, pr_burst_wall, _, _ = replay(1, None, npb)
+    pr_cap_qps = float(nqs[:npb].sum()) / pr_burst_wall
+    # the Poisson trace at a fixed request rate, against both
+    rate = float(args.serve_rate)
     arrivals = np.cumsum(expo / rate)
     b_ms, b_wall, b_res, b_batches = replay(0, arrivals, R)
-    # one launch sequence per request, same arrivals (first second of them)
     npr = int(min(R, max(50, rate * 1.0)))
     p_ms, p_wall, p_res, p_batches = replay(1, arrivals, npr)
     same = all(np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
@@ -222,7 +235,10 @@ def serve_bench(args, ck, datagen, torch, np):
     out = {"serve_trees": T, "serve_points": N, "serve_requests": R,
            "serve_queries": int(nqs.sum()),
            "serve_saturation_queries_per_s": round(cap_qps, 1),
+           "per_request_saturation_queries_per_s": round(pr_cap_qps, 1),
+           "serve_vs_per_request_saturation": round(cap_qps / pr_cap_qps, 2),
            "serve_rate_req_per_s": round(rate, 1),
+           "serve_offered_queries_per_s": round(rate * float(nqs.mean()), 1),
            "serve_p50_ms": round(float(np.percentile(b_ms, 50)), 3),
            "serve_p99_ms": round(float(np.percentile(b_ms, 99)), 3),
            "serve_queries_per_s": round(float(nqs.sum()) / b_wall, 1),
@@ -232,10 +248,11 @@ def serve_bench(args, ck, datagen, torch, np):
            "per_request_p99_ms": round(float(np.percentile(p_ms, 99)), 3),
            "per_request_queries_per_s": round(float(nqs[:npr].sum()) / p_wall, 1),
            "serve_equals_per_request": bool(same)}
-    print("[bench] serve: saturation %.1fM q/s, rate %.0f req/s; batched p50 %.2f ms p99 %.2f ms "
-          "(%d batches); per-request p50 %.2f ms p99 %.2f ms; identical: %s"
-          % (cap_qps / 1e6, rate, out["serve_p50_ms"], out["serve_p99_ms"], b_batches,
-             out["per_request_p50_ms"], out["per_request_p99_ms"], same),
+    print("[bench] serve: saturation %.1fM q/s batched vs %.1fM q/s per request; at %.0f req/s "
+          "batched p50 %.2f ms p99 %.2f ms (%d batches); per-request p50 %.2f ms p99 %.2f ms; "
+          "identical: %s"
+          % (cap_qps / 1e6, pr_cap_qps / 1e6, rate, out["serve_p50_ms"], out["serve_p99_ms"],
+             b_batches, out["per_request_p50_ms"], out["per_request_p99_ms"], same),
           file=sys.stderr, flush=True)
     f.close()
     del pts
@@ -262,6 +279,8 @@ def main():
     ap.add_argument("--serve-trees", type=int, default=512)
     ap.add_argument("--serve-requests", type=int, default=6000)
     ap.add_argument("--serve-budget-mb", type=int, default=512)
+    ap.add_argument("--serve-rate", type=float, default=2000.0,
+                    help="Poisson request rate (requests/s) of the serving trace")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--probe", action="store_true",
                     help="for hardware-counter runs: one 1 GiB device copy (the counters' "

@@ -371,15 +371,25 @@ class Server:
                                         _ptr(world_bounds), traversal, budget_bytes,
                                         max_requests_per_batch, _stream(stream)))
 
-    def submit(self, tree, queries, k=0, cutoff=math.inf):
+    def submit(self, tree, queries, k=0, cutoff=math.inf, out=None):
         """queries: float32 numpy (nq, D).  Returns the ticket; results are
-        in result(ticket) once polled complete."""
+        in result(ticket) once polled complete.  out: optional (ids, dist2)
+        host arrays of nq*max(1, k) int32 / float32 elements to receive the
+        results (reused buffers spare the page faults of fresh ones)."""
         import numpy as np
         q = np.ascontiguousarray(queries, dtype=np.float32)
         nq = len(q)
         w = max(1, k)
-        ids = np.empty((nq, w) if k else nq, np.int32)
-        d2 = np.empty((nq, w) if k else nq, np.float32)
+        if out is None:
+            ids = np.empty((nq, w) if k else nq, np.int32)
+            d2 = np.empty((nq, w) if k else nq, np.float32)
+        else:
+            ids, d2 = out
+            if (ids.dtype != np.int32 or d2.dtype != np.float32 or ids.size != nq * w
+                    or d2.size != nq * w or not ids.flags.c_contiguous
+                    or not d2.flags.c_contiguous):
+                raise ValueError("out must be C-contiguous int32/float32 arrays of "
+                                 "nq*max(1,k) elements")
         t = ctypes.c_uint64(0)
         _check(lib().cukd_server_submit(self._h, tree, q.ctypes.data, nq, k, cutoff,
                                         ids.ctypes.data, d2.ctypes.data, ctypes.byref(t)))

@@ -61,6 +61,103 @@ struct Slot {
   bool busy = false;
 };
 
+// Helper threads for a batch's host-side copies (queries into the pinned
+// slot, results out to the callers' buffers): one thread copies at a small
+// fraction of the host's memory bandwidth, and these copies bound the
+// server's throughput.  run() splits the copies into chunks that the
+// helpers and the calling thread take in turn; it returns when every chunk
+// is copied and no helper is still inside the generation.
+class CopyPool {
+ public:
+  struct Job {
+    void *dst;
+    const void *src;
+    size_t bytes;
+  };
+  explicit CopyPool(int helpers)
+  {
+    for (int i = 0; i < helpers; i++) th_.emplace_back([this] { helper(); });
+  }
+  ~CopyPool()
+  {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_) t.join();
+  }
+  void run(const std::vector<Job> &jobs)
+  {
+    enum { CHUNK = 1 << 20, SERIAL = 256 << 10 };
+    size_t total = 0;
+    for (const Job &j : jobs) total += j.bytes;
+    if (th_.empty() || total < SERIAL) {
+      for (const Job &j : jobs) memcpy(j.dst, j.src, j.bytes);
+      return;
+    }
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_done_.wait(lk, [&] { return busy_ == 0; });
+    chunks_.clear();
+    for (const Job &j : jobs)
+      for (size_t o = 0; o < j.bytes; o += CHUNK)
+        chunks_.push_back(Job{(uint8_t *)j.dst + o, (const uint8_t *)j.src + o,
+                              j.bytes - o < (size_t)CHUNK ? j.bytes - o : (size_t)CHUNK});
+    n_ = chunks_.size();
+    next_.store(0);
+    done_.store(0);
+    gen_++;
+    lk.unlock();
+    cv_.notify_all();
+    drain();
+    lk.lock();
+    cv_done_.wait(lk, [&] { return done_.load() == n_ && busy_ == 0; });
+  }
+
+ private:
+  void drain()
+  {
+    for (;;) {
+      const size_t i = next_.fetch_add(1);
+      if (i >= n_) break;
+      memcpy(chunks_[i].dst, chunks_[i].src, chunks_[i].bytes);
+      done_.fetch_add(1);
+    }
+  }
+  void helper()
+  {
+    uint64_t seen = 0;
+    for (;;) {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+      if (stop_) return;
+      seen = gen_;
+      busy_++;
+      lk.unlock();
+      drain();
+      lk.lock();
+      busy_--;
+      if (busy_ == 0) cv_done_.notify_all();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, cv_done_;
+  std::vector<Job> chunks_;
+  size_t n_ = 0;
+  std::atomic<size_t> next_{0}, done_{0};
+  int busy_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
+int copy_helpers()
+{
+  const unsigned hc = std::thread::hardware_concurrency();
+  const int h = hc > 2 ? (int)(hc / 2) - 1 : 0;
+  return h < 7 ? h : 7;
+}
+
 }  // namespace
 
 }  // namespace cukd_amd
@@ -87,6 +184,7 @@ struct cukd_server {
   int error = 0;
   char error_msg[256] = {0};
   cukd_server_stats stats{};
+  CopyPool *copies = nullptr;
   std::thread worker;
 };
 
@@ -136,6 +234,7 @@ int launch_batch(cukd_server *S, Slot &sl)
     return CUKD_ERR_INVALID;
   }
   sl.used = o;
+  std::vector<CopyPool::Job> jobs;
   for (size_t c = 0; c < cls.size(); c++) {
     float *hq = (float *)(sl.host + lay[c].q);
     int32_t *so = (int32_t *)(sl.host + lay[c].so);
@@ -144,7 +243,7 @@ int launch_batch(cukd_server *S, Slot &sl)
     const size_t kk = cls[c].k > 0 ? (size_t)cls[c].k : 1;
     for (int i : cls[c].members) {
       const Request *r = sl.reqs[i];
-      memcpy(hq + (size_t)acc * D, r->q, (size_t)r->nq * D * 4);
+      jobs.push_back(CopyPool::Job{hq + (size_t)acc * D, r->q, (size_t)r->nq * D * 4});
       so[s] = acc;
       stt[s] = r->tree;
       sl.ids_off[i] = lay[c].ids + (size_t)acc * kk * 4;
@@ -154,6 +253,7 @@ int launch_batch(cukd_server *S, Slot &sl)
     }
     so[s] = acc;
   }
+  S->copies->run(jobs);
   hipError_t e = hipMemcpyAsync(sl.dev, sl.host, res_begin, hipMemcpyHostToDevice, S->stream);
   for (size_t c = 0; c < cls.size() && e == hipSuccess; c++) {
     const float *dq = (const float *)(sl.dev + lay[c].q);
@@ -192,17 +292,20 @@ int launch_batch(cukd_server *S, Slot &sl)
 int finish_batch(cukd_server *S, Slot &sl)
 {
   const hipError_t e = hipEventSynchronize(sl.done);
-  const Clock::time_point now = Clock::now();
   const size_t nreq = sl.reqs.size();
-  for (size_t i = 0; i < nreq; i++) {
-    Request *r = sl.reqs[i];
-    const size_t kk = r->k > 0 ? (size_t)r->k : 1;
-    if (e == hipSuccess) {
-      memcpy(r->ids, sl.host + sl.ids_off[i], (size_t)r->nq * kk * 4);
-      if (r->dist2) memcpy(r->dist2, sl.host + sl.d2_off[i], (size_t)r->nq * kk * 4);
+  if (e == hipSuccess) {
+    std::vector<CopyPool::Job> jobs;
+    for (size_t i = 0; i < nreq; i++) {
+      const Request *r = sl.reqs[i];
+      const size_t kk = r->k > 0 ? (size_t)r->k : 1;
+      jobs.push_back(CopyPool::Job{r->ids, sl.host + sl.ids_off[i], (size_t)r->nq * kk * 4});
+      if (r->dist2)
+        jobs.push_back(CopyPool::Job{r->dist2, sl.host + sl.d2_off[i], (size_t)r->nq * kk * 4});
     }
-    r->t_done = now;
+    S->copies->run(jobs);
   }
+  const Clock::time_point now = Clock::now();
+  for (size_t i = 0; i < nreq; i++) sl.reqs[i]->t_done = now;
   std::lock_guard<std::mutex> lk(S->mu);
   for (size_t i = 0; i < nreq; i++) {
     Request *r = sl.reqs[i];
@@ -324,6 +427,7 @@ int cukd_server_create(cukd_server **server, const cukd_forest *forest, const vo
       return CUKD_ERR_ALLOC;
     }
   }
+  S->copies = new CopyPool(copy_helpers());
   S->worker = std::thread(worker_main, S);
   *server = S;
   return CUKD_OK;
@@ -409,6 +513,7 @@ int cukd_server_destroy(cukd_server *S)
   }
   S->cv_work.notify_all();
   S->worker.join();  // drains every pending request first
+  delete S->copies;
   for (int i = 0; i < 2; i++) {
     (void)hipFree(S->slot[i].dev);
     (void)hipHostFree(S->slot[i].host);

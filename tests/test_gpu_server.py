@@ -143,3 +143,29 @@ def test_budget_and_argument_errors(served_forest):
     tickets = [srv.submit(i % (len(off) - 1), datagen.uniform(500, 3, i), 4) for i in range(20)]
     srv.close()
     assert all(t in srv._keep for t in tickets)
+
+
+def test_caller_result_buffers(served_forest):
+    """submit(out=...) writes into the caller's arrays, with the same results
+    as server-allocated ones; wrong shapes are refused."""
+    f, pts, off, lay = served_forest
+    reqs = _requests(off, 12, 11)
+    srv = ck.Server(f, pts, budget_bytes=32 << 20)
+    ref = _serve(srv, reqs)
+    outs, tickets = [], []
+    for tree, q, k, cutoff in reqs:
+        n = len(q) * max(1, k)
+        outs.append((np.full(n, -5, np.int32), np.full(n, -5.0, np.float32)))
+        tickets.append(srv.submit(tree, q, k, cutoff, out=outs[-1]))
+    pending = set(tickets)
+    while pending:
+        for c in srv.poll(wait_for=1):
+            assert c.status == 0
+            pending.discard(c.ticket)
+    for t, (ids, d2), (rids, rd2) in zip(tickets, outs, ref):
+        got_ids, got_d2 = srv.result(t)
+        assert got_ids is ids and got_d2 is d2
+        assert np.array_equal(ids, rids.ravel()) and np.array_equal(d2, rd2.ravel())
+    with pytest.raises(ValueError):
+        srv.submit(0, reqs[0][1], 4, out=(np.zeros(3, np.int32), np.zeros(3, np.float32)))
+    srv.close()
