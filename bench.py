@@ -327,8 +327,8 @@ def main():
         # the queries in their Morton launch order, so that the kNN and FCP
         # dispatches are exactly the ones the roofline figure times
         qs = queries[ck.query_order(queries).long()].contiguous()
-        ck.knn(tree, lay, qs, args.k, out=(ids, d2, None))
-        ck.fcp(tree, lay, qs)
+        ck.knn(tree, lay, qs, args.k, out=(ids, d2, None), flags=ck.QUERY_PRESORTED)
+        ck.fcp(tree, lay, qs, flags=ck.QUERY_PRESORTED)
         torch.cuda.synchronize()
         print("[bench] probe done", file=sys.stderr, flush=True)
         return
@@ -417,25 +417,26 @@ def main():
     visits = int(counter.item())
     knn_bytes = nq * (S + k * 8 + 4) + visits * S
     # the kNN kernel alone: the same queries already in the Morton order the
-    # step launches them in, event-timed on the launch stream
+    # step launches them in (and launched with its shape), event-timed on
+    # the launch stream
     qs = queries[ck.query_order(queries).long()].contiguous()
     for _ in range(args.warmup):
-        ck.knn(tree, lay, qs, k, out=out)
+        ck.knn(tree, lay, qs, k, out=out, flags=ck.QUERY_PRESORTED)
     torch.cuda.synchronize()
     e0.record(stream)
     for _ in range(args.steps):
-        ck.knn(tree, lay, qs, k, out=out)
+        ck.knn(tree, lay, qs, k, out=out, flags=ck.QUERY_PRESORTED)
     e1.record(stream)
     torch.cuda.synchronize()
     kernel_ms = e0.elapsed_time(e1) / args.steps
     # the same launch with one query per lane (the kernel the default
     # two-per-lane one replaces), for comparison
     for _ in range(args.warmup):
-        ck.knn(tree, lay, qs, k, out=out, flags=ck.QUERY_ONE_PER_LANE)
+        ck.knn(tree, lay, qs, k, out=out, flags=ck.QUERY_ONE_PER_LANE | ck.QUERY_PRESORTED)
     torch.cuda.synchronize()
     e0.record(stream)
     for _ in range(args.steps):
-        ck.knn(tree, lay, qs, k, out=out, flags=ck.QUERY_ONE_PER_LANE)
+        ck.knn(tree, lay, qs, k, out=out, flags=ck.QUERY_ONE_PER_LANE | ck.QUERY_PRESORTED)
     e1.record(stream)
     torch.cuda.synchronize()
     one_lane_ms = e0.elapsed_time(e1) / args.steps

@@ -38,6 +38,7 @@ LIST_FIXED, LIST_HEAP = 0, 1
 QUERY_AUTOTUNE = 2
 QUERY_SORT = 1
 QUERY_ONE_PER_LANE = 4
+QUERY_PRESORTED = 8
 
 # every entry point declared in include/cukd_amd.h
 EXPORTED_SYMBOLS = (

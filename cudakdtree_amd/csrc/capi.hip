@@ -338,6 +338,7 @@ static int run_queries(QueryArgs a, int D, int traversal, int flags, const MemRe
   if (e == hipSuccess) {
     a.block = cfg.block;
     a.lanes = two_ok ? cfg.lanes : 1;
+    a.presorted = (flags & CUKD_QUERY_PRESORTED) != 0;
     a.order = cfg.sort ? make_query_order(otmp, a.queries, a.nq, D, stream) : nullptr;
     e = launch_query(D, traversal, a, stream);
   }

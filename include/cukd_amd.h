@@ -95,6 +95,10 @@ typedef struct cukd_mem_resource {
                              4 GiB) steps two queries per lane, so twice as
                              many node loads are in flight; results are
                              identical either way */
+#define CUKD_QUERY_PRESORTED 8 /* the queries are already in Morton order
+                             (e.g. permuted by cukd_query_order): launch
+                             with the shape of a CUKD_QUERY_SORT launch,
+                             without ordering them again */
 
 /* Builds a left-balanced k-d tree in place over d_points[0..n).
  * Replaces template<data_t,traits> buildTree / buildTree_thrust /

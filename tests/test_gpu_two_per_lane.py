@@ -131,3 +131,30 @@ def test_bench_shape_two_per_lane_sorted():
     q = datagen.uniform(20000, 3, 2)
     check_knn(*s, q, 8, float("inf"), lists=(ck.LIST_FIXED,), methods=SB,
               flags=TWO | ck.QUERY_SORT)
+
+
+def test_large_sorted_and_presorted_launches_agree():
+    """Batches of 2^19 queries and more launch the two-query kernel with
+    fewer resident waves when sorted or marked presorted: results equal the
+    plain launch (and the oracle on a subset)."""
+    s = _setup(distribution("uniform", 200_000, 3, seed=51))
+    lay, tree, box, dt, db = s
+    q = datagen.uniform(600_000, 3, 52)
+    dq = torch.from_numpy(q).cuda()
+    order = ck.query_order(dq).long()
+    qs = dq[order].contiguous()
+    for k in (1, 8):
+        plain = ck.knn(dt, lay, qs, k)
+        pres = ck.knn(dt, lay, qs, k, flags=ck.QUERY_PRESORTED)
+        srt = ck.knn(dt, lay, dq, k, flags=ck.QUERY_SORT)
+        torch.cuda.synchronize()
+        for a, b, c in zip(plain, pres, srt):
+            assert torch.equal(a, b)
+            assert torch.equal(a, c[order])
+    fp = ck.fcp(dt, lay, qs)
+    fs = ck.fcp(dt, lay, qs, flags=ck.QUERY_PRESORTED)
+    torch.cuda.synchronize()
+    assert torch.equal(fp[0], fs[0]) and torch.equal(fp[1], fs[1])
+    sub = qs[:20000].cpu().numpy()
+    check_knn(*s, sub, 8, float("inf"), lists=(ck.LIST_FIXED,), methods=SB,
+              flags=ck.QUERY_PRESORTED)
