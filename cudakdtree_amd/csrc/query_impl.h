@@ -370,6 +370,12 @@ __device__ inline void sb_step(R &res, const float (&q)[D], const float (&p)[D],
   }
 }
 
+// lists for which large sorted batches run with fewer resident waves (FCP
+// and k = 1 touch fewer nodes per query and measured faster at full
+// occupancy)
+template <class R> struct FewWaves { enum { ok = 0 }; };
+template <int K> struct FewWaves<cukd::FixedCandidateList<K>> { enum { ok = K >= 4 }; };
+
 // result types whose per-lane state is registers only (trees with
 // round-robin split dims only)
 template <class R> struct DualLane { enum { ok = 0 }; };
@@ -483,11 +489,12 @@ inline hipError_t launch_q(const QueryArgs &a, hipStream_t s)
       // workgroups are handed out in order (XCD-aware, along the Morton
       // order), so fewer waves in flight means each XCD's active queries
       // cover a smaller region and their nodes stay in its L2.  About 14
-      // waves per CU measured best at 1M queries (7-8 % faster than 32); the
-      // dynamic LDS, unused by the kernel, only limits how many workgroups
-      // share a CU.
+      // waves per CU measured best for kNN-8 at 1M queries (10 % faster than
+      // 32, with 2.5x less L2-miss traffic); the dynamic LDS, unused by the
+      // kernel, only limits how many workgroups share a CU.
       const size_t lds =
-          ((a.order || a.presorted) && nt == 128 && a.nq >= QUERY2_FEW_WAVES_MIN_NQ)
+          (FewWaves<R>::ok && (a.order || a.presorted) && nt == 128 &&
+           a.nq >= QUERY2_FEW_WAVES_MIN_NQ)
               ? LDS_PER_CU / 7
               : 0;
       if (a.fq.seg_off)

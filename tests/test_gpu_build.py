@@ -174,20 +174,20 @@ def test_ten_million_builds_repeat_and_agree():
     """At the bench size (10M float3, 24-bit uniform coordinates, so ties
     are common) the radix-select builder gives the same array on every run,
     and that array equals the independent bitonic builder's; both builders
-    equal buildTree_host at every size the oracle checks quickly."""
+    equal buildTree_host at every size the oracle checks quickly.  Many
+    repeats: a lost histogram count in the cross-workgroup select shows up
+    in only a few builds out of a hundred."""
     n = 10_000_000
     coords = datagen.uniform(n, 3, 1)
     lay = ck.float_points(3)
     src = torch.from_numpy(coords).cuda()
-    outs = []
-    for b in (ck.BUILDER_DEFAULT, ck.BUILDER_DEFAULT, ck.BUILDER_BITONIC, ck.BUILDER_DEFAULT,
-              ck.BUILDER_DEFAULT):
-        t = src.clone()
-        ck.build_tree(t, lay, b)
-        outs.append(t)
-    torch.cuda.synchronize()
-    for i, o in enumerate(outs[1:]):
-        assert torch.equal(o, outs[0]), "build %d differs" % (i + 1)
+    ref = src.clone()
+    ck.build_tree(ref, lay, ck.BUILDER_BITONIC)
+    t = torch.empty_like(src)
+    for i in range(24):
+        t.copy_(src)
+        ck.build_tree(t, lay, ck.BUILDER_DEFAULT)
+        assert torch.equal(t, ref), "default build %d differs from the bitonic build" % i
 
 
 @pytest.mark.slow
