@@ -1,12 +1,16 @@
 // server.hip -- a serving loop over a resident forest: requests (a tree, a
 // batch of host queries, FCP or kNN-k) arrive at any time from any thread;
 // one worker thread batches whatever is queued into one launch per query
-// kind (continuous batching), bounded by a device-memory budget, and
-// pipelines two batches (the next is packed and launched before the
-// previous one is waited for).  No counterpart in the reference, whose
+// kind (continuous batching), bounded by a device-memory budget, into one
+// of SERVER_SLOTS batch slots.  Each slot has a stream of its own and a
+// thread that waits for its batch and completes its requests, so batches
+// run side by side on the device: a batch held up by a few long traversals
+// does not hold up the ones admitted after it.  No counterpart in the
+// reference, whose
 // queries are per-thread device calls; every result equals the single-tree
 // cukd_fcp / cukd_knn result for the same queries.
 #include <atomic>
+#include <cstdarg>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -49,16 +53,24 @@ inline size_t request_bytes(int nq, int k, int D)
 
 inline size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
-// one in-flight batch
+enum { SERVER_SLOTS = 4 };
+
+// one batch: free -> packed by the worker -> launched -> completed by the
+// slot's own thread -> free (state guarded by the server's mutex)
+enum SlotState { SLOT_FREE = 0, SLOT_PACKING = 1, SLOT_LAUNCHED = 2 };
+
 struct Slot {
   uint8_t *dev = nullptr;   // queries | segment tables | results
   uint8_t *host = nullptr;  // pinned mirror
   size_t bytes = 0;
-  hipEvent_t done = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t kdone = nullptr;  // the batch's queries have run
+  hipEvent_t done = nullptr;   // its results are in 'host'
   std::vector<Request *> reqs;
   std::vector<size_t> ids_off, d2_off;  // per request: where its results are in the slot
-  size_t used = 0;
-  bool busy = false;
+  size_t res_begin = 0, used = 0;  // results: [res_begin, used) of the slot
+  int state = SLOT_FREE;
+  std::thread finisher;
 };
 
 // Helper threads for a batch's host-side copies (queries into the pinned
@@ -66,7 +78,8 @@ struct Slot {
 // fraction of the host's memory bandwidth, and these copies bound the
 // server's throughput.  run() splits the copies into chunks that the
 // helpers and the calling thread take in turn; it returns when every chunk
-// is copied and no helper is still inside the generation.
+// is copied and no helper is still inside the generation.  Calls from
+// several threads take turns.
 class CopyPool {
  public:
   struct Job {
@@ -96,6 +109,7 @@ class CopyPool {
       for (const Job &j : jobs) memcpy(j.dst, j.src, j.bytes);
       return;
     }
+    std::lock_guard<std::mutex> turn(call_mu_);
     std::unique_lock<std::mutex> lk(mu_);
     cv_done_.wait(lk, [&] { return busy_ == 0; });
     chunks_.clear();
@@ -141,6 +155,7 @@ class CopyPool {
     }
   }
   std::vector<std::thread> th_;
+  std::mutex call_mu_;  // one run() at a time
   std::mutex mu_;
   std::condition_variable cv_, cv_done_;
   std::vector<Job> chunks_;
@@ -173,10 +188,13 @@ struct cukd_server {
   int max_per_batch = 0;
   int device = 0;
   size_t slot_bytes = 0;
-  hipStream_t stream = nullptr;
-  Slot slot[2];
+  Slot slot[SERVER_SLOTS];
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
+  // cv_work: the worker (requests queued and a slot free, or stop);
+  // cv_slot: the slots' threads (a batch launched, or finish); cv_done:
+  // pollers and the worker's final drain
+  std::condition_variable cv_work, cv_slot, cv_done;
+  bool finish = false;  // the slots' threads exit once their slot is free
   std::deque<Request *> pending;
   std::deque<cukd_completion> done;
   uint64_t next_ticket = 1;
@@ -189,6 +207,18 @@ struct cukd_server {
 };
 
 namespace {
+
+// Records a batch error's message unless an earlier error is recorded
+// (called without the server's mutex; slots fail independently).
+void note_error(cukd_server *S, const char *fmt, ...)
+{
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (S->error) return;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(S->error_msg, sizeof(S->error_msg), fmt, ap);
+  va_end(ap);
+}
 
 // Packs the requests of one batch into a slot and launches it: one H2D
 // copy, one forest query launch per (k, cutoff) class, one D2H copy of the
@@ -230,9 +260,10 @@ int launch_batch(cukd_server *S, Slot &sl)
     lay[c].d2 = o; o = align_up(o + (size_t)lay[c].nq * kk * 4);
   }
   if (o > sl.bytes) {
-    snprintf(S->error_msg, sizeof(S->error_msg), "batch of %zu bytes exceeds the slot", o);
+    note_error(S, "batch of %zu bytes exceeds the slot", o);
     return CUKD_ERR_INVALID;
   }
+  sl.res_begin = res_begin;
   sl.used = o;
   std::vector<CopyPool::Job> jobs;
   for (size_t c = 0; c < cls.size(); c++) {
@@ -254,7 +285,7 @@ int launch_batch(cukd_server *S, Slot &sl)
     so[s] = acc;
   }
   S->copies->run(jobs);
-  hipError_t e = hipMemcpyAsync(sl.dev, sl.host, res_begin, hipMemcpyHostToDevice, S->stream);
+  hipError_t e = hipMemcpyAsync(sl.dev, sl.host, res_begin, hipMemcpyHostToDevice, sl.stream);
   for (size_t c = 0; c < cls.size() && e == hipSuccess; c++) {
     const float *dq = (const float *)(sl.dev + lay[c].q);
     const int32_t *dso = (const int32_t *)(sl.dev + lay[c].so);
@@ -264,34 +295,37 @@ int launch_batch(cukd_server *S, Slot &sl)
     int rc;
     if (cls[c].k == 0)
       rc = cukd_forest_fcp(S->traversal, S->forest, S->points, S->world_bounds, dq, lay[c].nq,
-                           lay[c].nseg, dso, dst, cls[c].cutoff, 0.f, ids, d2, 0, S->stream);
+                           lay[c].nseg, dso, dst, cls[c].cutoff, 0.f, ids, d2, 0, sl.stream);
     else
       rc = cukd_forest_knn(S->traversal, CUKD_LIST_FIXED, cls[c].k, S->forest, S->points,
                            S->world_bounds, dq, lay[c].nq, lay[c].nseg, dso, dst, cls[c].cutoff,
-                           ids, d2, nullptr, 0, S->stream);
+                           ids, d2, nullptr, 0, sl.stream);
     if (rc != CUKD_OK) {
-      snprintf(S->error_msg, sizeof(S->error_msg), "batch launch: %s", cukd_last_error());
+      note_error(S, "batch launch: %s", cukd_last_error());
       return rc;
     }
   }
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(sl.host + res_begin, sl.dev + res_begin, o - res_begin,
-                       hipMemcpyDeviceToHost, S->stream);
-  if (e == hipSuccess) e = hipEventRecord(sl.done, S->stream);
+  // the results' copy is queued by the slot's thread once the queries have
+  // run: copies queued behind a long batch's kernels would hold up the
+  // copies of the batches launched after it
+  if (e == hipSuccess) e = hipEventRecord(sl.kdone, sl.stream);
   if (e != hipSuccess) {
-    snprintf(S->error_msg, sizeof(S->error_msg), "batch copy: %s", hipGetErrorString(e));
+    note_error(S, "batch copy: %s", hipGetErrorString(e));
     return CUKD_ERR_HIP;
   }
-  sl.busy = true;
-  S->stats.batches++;
   return CUKD_OK;
 }
 
-// Waits for a slot's batch, copies every request's results to its caller
-// and records its completion.
+// Waits for a slot's batch, copies its results back and on to every
+// request's caller, and records the completions.
 int finish_batch(cukd_server *S, Slot &sl)
 {
-  const hipError_t e = hipEventSynchronize(sl.done);
+  hipError_t e = hipEventSynchronize(sl.kdone);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(sl.host + sl.res_begin, sl.dev + sl.res_begin, sl.used - sl.res_begin,
+                       hipMemcpyDeviceToHost, sl.stream);
+  if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
+  if (e == hipSuccess) e = hipEventSynchronize(sl.done);
   const size_t nreq = sl.reqs.size();
   if (e == hipSuccess) {
     std::vector<CopyPool::Job> jobs;
@@ -321,7 +355,8 @@ int finish_batch(cukd_server *S, Slot &sl)
     delete r;
   }
   sl.reqs.clear();
-  sl.busy = false;
+  sl.state = SLOT_FREE;
+  S->cv_work.notify_one();
   if (e != hipSuccess && !S->error) {
     S->error = CUKD_ERR_HIP;
     snprintf(S->error_msg, sizeof(S->error_msg), "batch failed: %s", hipGetErrorString(e));
@@ -330,18 +365,26 @@ int finish_batch(cukd_server *S, Slot &sl)
   return e == hipSuccess ? CUKD_OK : CUKD_ERR_HIP;
 }
 
+// The worker: admits queued requests into a free slot and launches the
+// batch on the slot's stream; the slot's own thread completes it.
 void worker_main(cukd_server *S)
 {
   (void)hipSetDevice(S->device);
-  int cur = 0;
-  int inflight = -1;
   for (;;) {
-    Slot &sl = S->slot[cur];
+    int si = -1;
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      if (inflight < 0)
-        S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty(); });
-      if (S->pending.empty() && inflight < 0 && S->stop) break;
+      auto free_slot = [&] {
+        for (int i = 0; i < SERVER_SLOTS; i++)
+          if (S->slot[i].state == SLOT_FREE) return i;
+        return -1;
+      };
+      S->cv_work.wait(lk, [&] {
+        return (S->stop && S->pending.empty()) || (!S->pending.empty() && free_slot() >= 0);
+      });
+      if (S->pending.empty()) break;  // stopping, and every request admitted
+      si = free_slot();
+      Slot &sl = S->slot[si];
       // admit requests in arrival order while they fit the slot
       size_t bytes = 0;
       const Clock::time_point now = Clock::now();
@@ -354,29 +397,56 @@ void worker_main(cukd_server *S)
         sl.reqs.push_back(r);
         S->pending.pop_front();
       }
+      sl.state = SLOT_PACKING;
     }
-    if (!sl.reqs.empty()) {
-      if (launch_batch(S, sl) != CUKD_OK) {
-        // fail every request of the batch
-        std::lock_guard<std::mutex> lk(S->mu);
-        for (Request *r : sl.reqs) {
-          cukd_completion c;
-          c.ticket = r->ticket;
-          c.status = CUKD_ERR_HIP;
-          c.num_queries = r->nq;
-          c.queue_ms = c.total_ms = 0.f;
-          S->done.push_back(c);
-          delete r;
-        }
-        sl.reqs.clear();
-        if (!S->error) S->error = CUKD_ERR_HIP;
-        S->cv_done.notify_all();
-        continue;
+    Slot &sl = S->slot[si];
+    const int rc = launch_batch(S, sl);
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (rc != CUKD_OK) {
+      // fail every request of the batch
+      for (Request *r : sl.reqs) {
+        cukd_completion c;
+        c.ticket = r->ticket;
+        c.status = CUKD_ERR_HIP;
+        c.num_queries = r->nq;
+        c.queue_ms = c.total_ms = 0.f;
+        S->done.push_back(c);
+        delete r;
       }
+      sl.reqs.clear();
+      sl.state = SLOT_FREE;
+      if (!S->error) S->error = CUKD_ERR_HIP;
+      S->cv_done.notify_all();
+      continue;
     }
-    if (inflight >= 0) finish_batch(S, S->slot[inflight]);
-    inflight = sl.busy ? cur : -1;
-    if (sl.busy) cur ^= 1;
+    sl.state = SLOT_LAUNCHED;
+    S->stats.batches++;
+    S->cv_slot.notify_all();
+  }
+  // every admitted batch completes before the slots' threads are released
+  std::unique_lock<std::mutex> lk(S->mu);
+  S->cv_work.wait(lk, [&] {
+    for (int i = 0; i < SERVER_SLOTS; i++)
+      if (S->slot[i].state != SLOT_FREE) return false;
+    return true;
+  });
+  S->finish = true;
+  S->cv_slot.notify_all();
+}
+
+// A slot's thread: waits for each batch launched into the slot, copies its
+// results to the callers and records the completions.
+void slot_main(cukd_server *S, int si)
+{
+  (void)hipSetDevice(S->device);
+  Slot &sl = S->slot[si];
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_slot.wait(lk, [&] { return sl.state == SLOT_LAUNCHED || S->finish; });
+      if (sl.state != SLOT_LAUNCHED) return;
+    }
+    finish_batch(S, sl);
   }
 }
 
@@ -406,28 +476,40 @@ int cukd_server_create(cukd_server **server, const cukd_forest *forest, const vo
   S->world_bounds = d_world_bounds;
   S->traversal = traversal;
   S->max_per_batch = max_requests_per_batch;
-  S->stream = stream;
   S->D = forest_num_dims(forest);
   if (hipGetDevice(&S->device) != hipSuccess) S->device = 0;
-  S->slot_bytes = budget_bytes / 2;
-  for (int i = 0; i < 2; i++) {
+  S->slot_bytes = budget_bytes / SERVER_SLOTS;
+  // the slots' streams start after the work queued on 'stream' so far
+  // (the forest's build, typically)
+  hipEvent_t ready = nullptr;
+  hipError_t e = hipEventCreateWithFlags(&ready, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(ready, stream);
+  for (int i = 0; i < SERVER_SLOTS && e == hipSuccess; i++) {
     Slot &sl = S->slot[i];
     sl.bytes = S->slot_bytes;
-    hipError_t e = hipMalloc((void **)&sl.dev, sl.bytes);
+    e = hipMalloc((void **)&sl.dev, sl.bytes);
     if (e == hipSuccess) e = hipHostMalloc((void **)&sl.host, sl.bytes, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
-    if (e != hipSuccess) {
-      set_last_error("server set-up failed: %s", hipGetErrorString(e));
-      for (int j = 0; j <= i; j++) {
-        if (S->slot[j].dev) (void)hipFree(S->slot[j].dev);
-        if (S->slot[j].host) (void)hipHostFree(S->slot[j].host);
-        if (S->slot[j].done) (void)hipEventDestroy(S->slot[j].done);
-      }
-      delete S;
-      return CUKD_ERR_ALLOC;
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.kdone, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamWaitEvent(sl.stream, ready, 0);
+  }
+  if (ready) (void)hipEventDestroy(ready);
+  if (e != hipSuccess) {
+    set_last_error("server set-up failed: %s", hipGetErrorString(e));
+    for (int j = 0; j < SERVER_SLOTS; j++) {
+      Slot &sl = S->slot[j];
+      if (sl.stream) (void)hipStreamDestroy(sl.stream);
+      if (sl.dev) (void)hipFree(sl.dev);
+      if (sl.host) (void)hipHostFree(sl.host);
+      if (sl.done) (void)hipEventDestroy(sl.done);
+      if (sl.kdone) (void)hipEventDestroy(sl.kdone);
     }
+    delete S;
+    return CUKD_ERR_ALLOC;
   }
   S->copies = new CopyPool(copy_helpers());
+  for (int i = 0; i < SERVER_SLOTS; i++) S->slot[i].finisher = std::thread(slot_main, S, i);
   S->worker = std::thread(worker_main, S);
   *server = S;
   return CUKD_OK;
@@ -512,12 +594,16 @@ int cukd_server_destroy(cukd_server *S)
     S->stop = true;
   }
   S->cv_work.notify_all();
-  S->worker.join();  // drains every pending request first
+  S->worker.join();  // serves every pending request first
+  for (int i = 0; i < SERVER_SLOTS; i++) S->slot[i].finisher.join();
   delete S->copies;
-  for (int i = 0; i < 2; i++) {
-    (void)hipFree(S->slot[i].dev);
-    (void)hipHostFree(S->slot[i].host);
-    (void)hipEventDestroy(S->slot[i].done);
+  for (int i = 0; i < SERVER_SLOTS; i++) {
+    Slot &sl = S->slot[i];
+    (void)hipStreamDestroy(sl.stream);
+    (void)hipFree(sl.dev);
+    (void)hipHostFree(sl.host);
+    (void)hipEventDestroy(sl.done);
+    (void)hipEventDestroy(sl.kdone);
   }
   const int err = S->error;
   delete S;

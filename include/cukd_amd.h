@@ -294,13 +294,15 @@ int cukd_forest_reserve_queries(cukd_forest *forest, int32_t max_queries,
  * No counterpart in the reference.  Requests (one tree, nq host queries,
  * FCP or kNN-k) are submitted from any thread; a worker thread packs
  * whatever is queued into one batch (admitted in arrival order while the
- * batch fits half of budget_bytes of device memory, at most
- * max_requests_per_batch requests, <= 0 for no limit), launches one
- * forest query per (k, cutoff) class, copies results back and completes
- * the requests; the next batch is launched before the previous one is
- * waited for.  max_requests_per_batch = 1 serves each request with its own
- * launches.  Results equal cukd_fcp / cukd_knn (fixed list) on the
- * request's tree alone. */
+ * batch fits a quarter of budget_bytes of device memory, at most
+ * max_requests_per_batch requests, <= 0 for no limit) and launches one
+ * forest query per (k, cutoff) class.  Up to four batches are in flight,
+ * each on a stream of its own, and each is completed (results copied back,
+ * requests completed) as soon as it finishes, so a batch slowed by a few
+ * long traversals does not delay the batches launched after it.
+ * max_requests_per_batch = 1 serves each request with its own launches.
+ * Results equal cukd_fcp / cukd_knn (fixed list) on the request's tree
+ * alone. */
 typedef struct cukd_server cukd_server;
 
 typedef struct cukd_completion {
@@ -319,8 +321,10 @@ typedef struct cukd_server_stats {
 } cukd_server_stats;
 
 /* d_points / d_world_bounds: the built forest's arrays (bounds required
- * for CUKD_CCT).  Allocates two device batch buffers and their pinned host
- * mirrors (budget_bytes in total) and starts the worker thread. */
+ * for CUKD_CCT).  Allocates four device batch buffers and their pinned host
+ * mirrors (budget_bytes in total), creates the batches' streams, which
+ * start after the work queued on 'stream' so far (the forest build, say),
+ * and starts the worker thread. */
 int cukd_server_create(cukd_server **server, const cukd_forest *forest,
                        const void *d_points, const void *d_world_bounds,
                        int traversal, size_t budget_bytes,

@@ -134,7 +134,7 @@ def test_budget_and_argument_errors(served_forest):
     f, pts, off, lay = served_forest
     srv = ck.Server(f, pts, budget_bytes=1 << 20)
     with pytest.raises(ck.CukdError):
-        srv.submit(0, datagen.uniform(100000, 3, 1), 16)  # larger than half the budget
+        srv.submit(0, datagen.uniform(100000, 3, 1), 16)  # larger than a quarter of the budget
     with pytest.raises(ck.CukdError):
         srv.submit(len(off) - 1, datagen.uniform(10, 3, 1), 4)  # no such tree
     with pytest.raises(ck.CukdError):
