@@ -538,17 +538,34 @@ __global__ __launch_bounds__(SEL_NT) void k_select_pass(
 // slot in the output, the rest to the two child subtrees of the next
 // level's compact layout (order inside a child is irrelevant: the next
 // level selects by explicit key).
+//
+// SW > 0 (elements of at most SW words, SW = 3 or 4): the tile's elements
+// and indices are staged in LDS and written out in destination order, so
+// consecutive lanes store consecutive words of one child run; in source
+// order (SW = 0, wider elements), the stores of a tile scatter over four
+// runs and the L2s take one request per few words.  The staging buffer is
+// sized for SW words so that 3-word points keep four workgroups per CU.
+template <int SW>
 __global__ __launch_bounds__(SEL_NT) void k_partition(
     TreeSet<LvlTree> set, int level, const uint32_t *key, const uint8_t *data,
     const uint32_t *idx, const SegState *st, uint8_t *ndata, uint32_t *nidx, uint32_t *nkey,
     uint32_t *cursors, uint8_t *out, cukd_layout L, int explicit_dim)
 {
+  enum { TILE = SEL_ITEMS * SEL_NT, STAGED = SW > 0 };
   __shared__ SegState s_st[2];
   __shared__ uint32_t s_cdim[4];
   __shared__ uint32_t s_cnt[SEL_ITEMS][SEL_NT / 64][4];
   __shared__ uint32_t s_base[4];
-  __shared__ int s_dst[SEL_ITEMS * SEL_NT];     // per tile element: child slot, or -(node+1) for the pivot
-  __shared__ uint8_t s_edim[SEL_ITEMS * SEL_NT];  // child key dim, or the pivot's split dim
+  __shared__ uint32_t s_tot[4];  // elements of each class in the tile
+  // source order: per tile element, child slot or -(node+1) for the pivot
+  __shared__ int s_dst[STAGED ? 1 : TILE];
+  __shared__ uint8_t s_edim[STAGED ? 1 : TILE];  // child key dim, or the pivot's split dim
+  // staged: the tile's words and indices, and the element at each
+  // destination-order position
+  __shared__ uint32_t s_tile[STAGED ? TILE * SW : 1];
+  __shared__ uint32_t s_idx[STAGED ? TILE : 1];
+  __shared__ uint16_t s_inv[STAGED ? TILE : 1];
+  __shared__ int s_pe[2];  // staged: tile element of each slot's pivot, or -1
   __shared__ LvlTree s_tree;
   if (threadIdx.x == 0)
     s_tree = find_tree(set, (int)blockIdx.x, [](const LvlTree &x) { return x.ptile0; });
@@ -586,6 +603,37 @@ __global__ __launch_bounds__(SEL_NT) void k_partition(
   __syncthreads();
   // tiles inside one subtree use slot 0 only
   const int seg1_begin = seg1 != seg0 ? g.offset(seg1) : g.n_elems;
+  const int W = L.stride >> 2;
+  const int nel = t1 - t0;
+  if constexpr (STAGED) {
+    // the tile's words and indices into LDS (loads issued before the
+    // classification's, all in flight together)
+    const uint32_t *src = (const uint32_t *)(data + (size_t)t0 * L.stride);
+    const int nw = nel * W;
+    uint32_t v[SW * SEL_ITEMS];
+#pragma unroll
+    for (int u = 0; u < SW * SEL_ITEMS; u++) {
+      const int i = (int)threadIdx.x + u * SEL_NT;
+      v[u] = i < nw ? src[i] : 0u;
+    }
+    uint32_t vi[SEL_ITEMS];
+#pragma unroll
+    for (int u = 0; u < SEL_ITEMS; u++) {
+      const int e = (int)threadIdx.x + u * SEL_NT;
+      vi[u] = e < nel ? idx[t0 + e] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < SW * SEL_ITEMS; u++) {
+      const int i = (int)threadIdx.x + u * SEL_NT;
+      if (i < nw) s_tile[i] = v[u];
+    }
+#pragma unroll
+    for (int u = 0; u < SEL_ITEMS; u++) {
+      const int e = (int)threadIdx.x + u * SEL_NT;
+      if (e < nel) s_idx[e] = vi[u];
+    }
+    if (threadIdx.x < 2) s_pe[threadIdx.x] = -1;
+  }
   // classify (all key loads in flight first; the element and index only
   // for subtrees still tied on a later word)
   uint32_t kv[SEL_ITEMS];
@@ -642,8 +690,62 @@ __global__ __launch_bounds__(SEL_NT) void k_partition(
       }
     const int s = (cc >> 1) ? seg1 : seg0;
     s_base[cc] = run ? atomicAdd(&cursors[2 * s + (cc & 1)], run) : 0u;
+    s_tot[cc] = run;
   }
   __syncthreads();
+  if constexpr (STAGED) {
+    // destination order: class c (slot*2 + side) holds positions
+    // [pre[c], pre[c+1]); position pos of class c goes to child slot
+    // cbase[c] + pos - pre[c]
+    int pre[5];
+    int cbase[4];
+    pre[0] = 0;
+    for (int c = 0; c < 4; c++) {
+      pre[c + 1] = pre[c] + (int)s_tot[c];
+      const int sg = (c >> 1) ? seg1 : seg0;
+      cbase[c] = gn.offset(2 * sg + (c & 1)) + (int)s_base[c];
+    }
+    for (int k = 0; k < SEL_ITEMS; k++) {
+      const int e = k * SEL_NT + (int)threadIdx.x;
+      if (cls[k] == 0xff) continue;
+      const int side = cls[k] & 3, slot = cls[k] >> 2;
+      if (side == 2) {
+        s_pe[slot] = e;
+        continue;
+      }
+      const int c = slot * 2 + side;
+      s_inv[pre[c] + (int)(s_cnt[k][wave][c] + wrank[k])] = (uint16_t)e;
+    }
+    __syncthreads();
+    const int npos = pre[4];
+    const int nw = npos * W;
+    const uint32_t magic = (uint32_t)((0x100000000ull + (unsigned)W - 1u) / (unsigned)W);  // pos = t / W
+    const int cw0 = L.point_offset >> 2;
+    uint32_t *nd = (uint32_t *)ndata;
+    for (int t = (int)threadIdx.x; t < nw; t += SEL_NT) {
+      const int pos = W == 1 ? t : (int)__umulhi((uint32_t)t, magic);
+      const int w = t - pos * W;
+      const int c = (pos >= pre[1]) + (pos >= pre[2]) + (pos >= pre[3]);
+      const int dst = cbase[c] + pos - pre[c];
+      const uint32_t v = s_tile[(int)s_inv[pos] * W + w];
+      nd[(size_t)dst * W + w] = v;
+      if (write_next_key && w == cw0 + (int)s_cdim[c]) nkey[dst] = ordered_key(v, isf);
+    }
+    for (int pos = (int)threadIdx.x; pos < npos; pos += SEL_NT) {
+      const int c = (pos >= pre[1]) + (pos >= pre[2]) + (pos >= pre[3]);
+      nidx[cbase[c] + pos - pre[c]] = s_idx[s_inv[pos]];
+    }
+    // the pivots to their node slots
+    for (int i = (int)threadIdx.x; i < 2 * W; i += SEL_NT) {
+      const int sl = i / W, w = i - sl * W;
+      const int e = s_pe[sl];
+      if (e < 0) continue;
+      const int node = (1 << g.L) - 1 + (sl ? seg1 : seg0);
+      ((uint32_t *)(out + (size_t)node * L.stride))[w] =
+          patch_dim_word(s_tile[e * W + w], w, L, (int)s_st[sl].dim);
+    }
+    return;
+  }
   // destinations, per tile element
   for (int k = 0; k < SEL_ITEMS; k++) {
     const int e = k * SEL_NT + (int)threadIdx.x;
@@ -664,8 +766,6 @@ __global__ __launch_bounds__(SEL_NT) void k_partition(
   // Copy word by word: the tile's elements are contiguous in the source,
   // so the loads are coalesced and independent; each element's words go to
   // its destination (child slot, or the pivot's node).
-  const int W = L.stride >> 2;
-  const int nel = t1 - t0;
   const int nw = nel * W;
   const uint32_t *src = (const uint32_t *)(data + (size_t)t0 * L.stride);
   const uint32_t magic = (uint32_t)((0x100000000ull + (unsigned)W - 1u) / (unsigned)W);  // e = i / W
@@ -1518,9 +1618,12 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
                            (const uint8_t *)dcur, (const uint32_t *)icur, st, hist, need, L,
                            expl ? 1 : 0, hcopies);
     }
-    hipLaunchKernelGGL(k_partition, dim3(P.lvl_ptiles[level]), dim3(SEL_NT), 0, s, set, level,
+    const int W = L.stride >> 2;
+    auto kp = W <= 3 ? k_partition<3> : (W == 4 ? k_partition<4> : k_partition<0>);
+    hipLaunchKernelGGL(kp, dim3(P.lvl_ptiles[level]), dim3(SEL_NT), 0, s, set, level,
                        (const uint32_t *)kcur, (const uint8_t *)dcur, (const uint32_t *)icur,
-                       (const SegState *)st, dnext, inext, knext, cursors, pts, L, expl ? 1 : 0);
+                       (const SegState *)st, dnext, inext, knext, cursors, pts, L,
+                       expl ? 1 : 0);
     uint8_t *t8 = dcur; dcur = dnext; dnext = t8;
     uint32_t *t32 = icur; icur = inext; inext = t32;
     t32 = kcur; kcur = knext; knext = t32;
