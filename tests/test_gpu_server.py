@@ -169,3 +169,25 @@ def test_caller_result_buffers(served_forest):
     with pytest.raises(ValueError):
         srv.submit(0, reqs[0][1], 4, out=(np.zeros(3, np.int32), np.zeros(3, np.float32)))
     srv.close()
+
+
+def test_server_starts_after_the_creation_stream(served_forest):
+    """The server's batch streams wait for the work queued on the stream
+    given at creation: a forest built there, with no host synchronisation
+    before the server is created, is served with the single-tree results."""
+    f, pts, off, lay = served_forest
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        pts2 = pts.clone()
+        f2 = ck.Forest(off, lay)
+        f2.build(pts2, stream=side)
+    srv = ck.Server(f2, pts2, budget_bytes=32 << 20, stream=side)
+    reqs = _requests(off, 16, 13)
+    got = _serve(srv, reqs)
+    srv.close()
+    torch.cuda.synchronize()
+    for (tree, q, k, cutoff), (ids, d2) in zip(reqs, got):
+        eids, ed2 = _expected(pts2, off, lay, tree, q, k, cutoff)
+        assert np.array_equal(ids, eids) and np.array_equal(d2.view(np.uint32), ed2.view(np.uint32))
+    f2.close()
