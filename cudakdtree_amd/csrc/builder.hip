@@ -1602,8 +1602,9 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
                        P.lvl_segs[level], st, cursors, need, level, D, expl ? 1 : 0);
     const int max_words = expl ? D + 1 : (level + 1 < D ? level + 1 : D) + 1;
     // histogram copies per subtree where many tiles add into one subtree
-    // (at least 32 tiles per copy, since the last tile reads every copy), as
-    // many as the buffer (sized for the level with the most subtrees) holds
+    // (at least 64 tiles per copy: the last tile reads every copy, and 32 or
+    // 16 tiles per copy measured slower at 10M points), as many as the
+    // buffer (sized for the level with the most subtrees) holds
     const int tiles_per_seg = P.lvl_stiles[level] / (P.lvl_segs[level] > 0 ? P.lvl_segs[level] : 1);
     int hcopies = 1;
     while (hcopies < SEL_HCOPIES && tiles_per_seg >= 64 * hcopies &&
