@@ -37,10 +37,12 @@
 
 namespace cukd_amd {
 
-enum { SEL_NT = 256, SEL_ITEMS = 8, SEL_BATCH = 8, DIGIT_MAX_BINS = 2048 };
-// select passes run 512 threads on each tile of keys: twice the waves of the
-// partition's workgroups per tile (a 10M build measured 5.27 ms against
-// 5.56 ms with 256 threads and 6.0 ms with 1024)
+// Partition workgroups: SEL_NT threads x SEL_ITEMS elements per tile (a
+// 10M float3 build measured 5.0 ms with 512 x 4, 5.2 ms with 256 x 8 and
+// 5.7 ms with 128 x 16: fewer registers per thread, more waves per CU).
+enum { SEL_NT = 512, SEL_ITEMS = 4, SEL_BATCH = 8, DIGIT_MAX_BINS = 2048 };
+// Select passes: 512 threads on each tile of keys (5.27 ms against 5.56 ms
+// with 256 threads and 6.0 ms with 1024, before the partition change).
 enum { SELECT_NT = 512 };
 // per subtree: DIGIT_MAX_BINS counters, then one bit per bin that some tile
 // touched in the current pass
