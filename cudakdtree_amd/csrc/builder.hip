@@ -1353,11 +1353,15 @@ inline int cap_class(int cap) { return 31 - __builtin_clz((unsigned)cap) - 9; } 
 // with enough finish workgroups to fill the GPU every tree takes the
 // largest capacity (fewest top levels); otherwise the capacity shrinks
 // until the tree alone has >= 256 finish workgroups.
-void tree_cap(int n, bool many, int &cap, int &L0)
+// Explicit-dim trees start from 4096-element subtrees: their bitonic finish
+// costs grow with log^2 of the subtree size, and a 10M float3 build
+// measured 9.6 ms against 11.1 ms with 8192 (one more global level).  The
+// rank finish of round-robin trees is fastest from 8192 (5.0 vs 5.2 ms).
+void tree_cap(int n, bool many, bool expl, int &cap, int &L0)
 {
   const int h = num_levels_for(n);
   L0 = 0;
-  for (cap = 8192; cap >= 512; cap >>= 1) {
+  for (cap = expl ? 4096 : 8192; cap >= 512; cap >>= 1) {
     const int lg = 31 - __builtin_clz((unsigned)cap);
     L0 = h - lg > 0 ? h - lg : 0;
     if (many || (1 << L0) >= 256 || cap == 512) break;
@@ -1415,7 +1419,7 @@ int make_forest_plan(ForestPlan &P, const long long *off, int ntrees, const cukd
   for (int t = 0; t < ntrees; t++) {
     const int n = (int)(off[t + 1] - off[t]);
     if (n < 1) continue;
-    tree_cap(n, true, cap[t], L0[t]);
+    tree_cap(n, true, P.expl, cap[t], L0[t]);
     blocks_max_cap += 1ll << L0[t];
   }
   const bool many = blocks_max_cap >= 512;
@@ -1423,7 +1427,7 @@ int make_forest_plan(ForestPlan &P, const long long *off, int ntrees, const cukd
   for (int t = 0; t < ntrees; t++) {
     const int n = (int)(off[t + 1] - off[t]);
     if (n < 1) continue;
-    tree_cap(n, many, cap[t], L0[t]);
+    tree_cap(n, many, P.expl, cap[t], L0[t]);
     P.max_L0 = L0[t] > P.max_L0 ? L0[t] : P.max_L0;
   }
   // segment states, numbered level by level
