@@ -8,6 +8,7 @@ The names mirror the reference's header API (cudaKDTree v1.0.1):
 
   buildTree / buildTree_thrust / buildTree_bitonic / buildTree_inPlace
       -> cukd::buildTree* (builder.h:108-151)
+  buildTree_host (host numpy array)    -> builder_thrust.h:114-118
   computeBounds                        -> builder_common.h:59-64
   stackBased.fcp / stackFree.fcp / cct.fcp      -> fcp.h:61-163
   stackBased.knn / stackFree.knn / cct.knn      -> knn.h:83-199
@@ -32,7 +33,9 @@ _lib = None
 OK = 0
 ERR_INVALID, ERR_HIP, ERR_NO_BOUNDS, ERR_ALLOC, ERR_UNSUPPORTED = -1, -2, -3, -4, -5
 SCALAR_F32, SCALAR_I32 = 0, 1
-BUILDER_DEFAULT, BUILDER_THRUST, BUILDER_BITONIC, BUILDER_INPLACE = 0, 1, 2, 3
+BUILDER_ID_DEFAULT, BUILDER_ID_THRUST, BUILDER_ID_BITONIC, BUILDER_ID_INPLACE = 0, 1, 2, 3
+BUILDER_DEFAULT, BUILDER_THRUST = BUILDER_ID_DEFAULT, BUILDER_ID_THRUST
+BUILDER_BITONIC, BUILDER_INPLACE = BUILDER_ID_BITONIC, BUILDER_ID_INPLACE
 STACK_BASED, STACK_FREE, CCT = 0, 1, 2
 LIST_FIXED, LIST_HEAP = 0, 1
 QUERY_AUTOTUNE = 2
@@ -42,7 +45,8 @@ QUERY_PRESORTED = 8
 
 # every entry point declared in include/cukd_amd.h
 EXPORTED_SYMBOLS = (
-    "cukd_build_tree", "cukd_build_tree_temp_bytes", "cukd_compute_bounds",
+    "cukd_build_tree", "cukd_build_tree_host", "cukd_build_tree_temp_bytes",
+    "cukd_compute_bounds",
     "cukd_fcp", "cukd_knn", "cukd_spatial_build", "cukd_spatial_free",
     "cukd_spatial_fcp", "cukd_spatial_knn", "cukd_set_traversal_stats",
     "cukd_knn_accumulate", "cukd_knn_extract", "cukd_query_order",
@@ -140,6 +144,7 @@ def lib():
     P, i32, f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float
     LP = ctypes.POINTER(Layout)
     L.cukd_build_tree.argtypes = [P, i32, LP, ctypes.c_int, P, P, P]
+    L.cukd_build_tree_host.argtypes = [P, i32, LP, P]
     L.cukd_build_tree_temp_bytes.argtypes = [i32, LP, ctypes.c_int,
                                              ctypes.POINTER(ctypes.c_size_t)]
     L.cukd_compute_bounds.argtypes = [P, P, i32, LP, P]
@@ -239,6 +244,24 @@ def buildTree_inPlace(points, layout, world_bounds=None, stream=None):
 
 
 buildTree_inplace = buildTree_inPlace
+
+
+def buildTree_host(points, layout, want_bounds=False):
+    """buildTree_host (builder_thrust.h:114-118): builds the tree in place
+    over a host numpy array of ``layout.stride``-byte rows (C-contiguous,
+    writeable).  Returns the host bounds box (lower[D] then upper[D]) when
+    ``want_bounds``, else None."""
+    if not (isinstance(points, np.ndarray) and points.flags.c_contiguous
+            and points.flags.writeable):
+        raise ValueError("points must be a writeable C-contiguous numpy array")
+    if points.nbytes % layout.stride:
+        raise ValueError("array size is not a multiple of the element stride")
+    box = np.zeros(2 * layout.num_dims, dtype=layout.scalar_dtype) if want_bounds else None
+    _check(lib().cukd_build_tree_host(points.ctypes.data_as(ctypes.c_void_p),
+                                      points.nbytes // layout.stride, ctypes.byref(layout),
+                                      None if box is None else box.ctypes.data_as(
+                                          ctypes.c_void_p)))
+    return box
 
 
 def build_temp_bytes(n, layout, builder=BUILDER_DEFAULT):

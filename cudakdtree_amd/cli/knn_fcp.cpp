@@ -257,7 +257,7 @@ int main(int ac, const char **av)
     std::cout << "calling builder..." << std::endl;
     double t0 = now_s();
     if (spatial) check(cukd_spatial_build(&st, d_pts, N, &L, lt, 0, nullptr), "spatial build");
-    else check(cukd_build_tree(d_pts, N, &L, CUKD_BUILDER_DEFAULT, d_bounds, 0, nullptr), "build");
+    else check(cukd_build_tree(d_pts, N, &L, CUKD_BUILDER_ID_DEFAULT, d_bounds, 0, nullptr), "build");
     hip_check(hipDeviceSynchronize(), "build");
     double t1 = now_s();
     std::cout << "done building tree, took " << pretty(t1 - t0) << "s" << std::endl;

@@ -62,6 +62,7 @@ int build_tree_impl(void *d_points, int n, const cukd_layout &L, void *d_bounds,
 size_t build_temp_bytes(int n, const cukd_layout &L);
 int build_bounds_only(void *d_bounds, const void *d_points, int n, const cukd_layout &L,
                       hipStream_t s);
+int screen_nan(const void *d_points, int n, const cukd_layout &L, hipStream_t stream);
 
 size_t query_order_temp_bytes(int nq);
 int32_t *make_query_order(void *tmp, const float *q, int nq, int D, hipStream_t s);
@@ -144,6 +145,8 @@ struct cukd_forest {
   bool has_mem = false;
   void *order_ws = nullptr;     // Morton-ordering scratch (cukd_forest_reserve_queries)
   size_t order_ws_bytes = 0;
+  hipStream_t order_stream = nullptr;  // the only stream order_ws is used on
+  std::mutex order_mu;                 // guards order_ws / order_stream
 };
 
 namespace cukd_amd {
@@ -170,13 +173,13 @@ int cukd_build_tree_temp_bytes(int32_t n, const cukd_layout *layout, int builder
   g_last_error[0] = 0;
   if (int rc = check_layout(layout)) return rc;
   if (!bytes) { set_last_error("bytes is null"); return CUKD_ERR_INVALID; }
-  if (builder < CUKD_BUILDER_DEFAULT || builder > CUKD_BUILDER_INPLACE) {
+  if (builder < CUKD_BUILDER_ID_DEFAULT || builder > CUKD_BUILDER_ID_INPLACE) {
     set_last_error("unknown builder %d", builder);
     return CUKD_ERR_INVALID;
   }
   if (n < 1) *bytes = 0;
-  else if (builder == CUKD_BUILDER_BITONIC) *bytes = build_tl_temp_bytes(n, *layout, true);
-  else if (builder == CUKD_BUILDER_INPLACE) *bytes = 0;
+  else if (builder == CUKD_BUILDER_ID_BITONIC) *bytes = build_tl_temp_bytes(n, *layout, true);
+  else if (builder == CUKD_BUILDER_ID_INPLACE) *bytes = 0;
   else *bytes = build_temp_bytes(n, *layout);
   return CUKD_OK;
 }
@@ -186,7 +189,7 @@ int cukd_build_tree(void *d_points, int32_t n, const cukd_layout *layout, int bu
 {
   g_last_error[0] = 0;
   if (int rc = check_layout(layout)) return rc;
-  if (builder < CUKD_BUILDER_DEFAULT || builder > CUKD_BUILDER_INPLACE) {
+  if (builder < CUKD_BUILDER_ID_DEFAULT || builder > CUKD_BUILDER_ID_INPLACE) {
     set_last_error("unknown builder %d", builder);
     return CUKD_ERR_INVALID;
   }
@@ -201,11 +204,51 @@ int cukd_build_tree(void *d_points, int32_t n, const cukd_layout *layout, int bu
                    "but no memory for world bounds provided");
     return CUKD_ERR_NO_BOUNDS;
   }
+  if (int rc = screen_nan(d_points, n, *layout, stream)) return rc;
   MemResource mr{mem};
-  if (builder == CUKD_BUILDER_BITONIC || builder == CUKD_BUILDER_INPLACE)
+  if (builder == CUKD_BUILDER_ID_BITONIC || builder == CUKD_BUILDER_ID_INPLACE)
     return build_tree_tl(d_points, n, *layout, d_world_bounds, stream, mr,
-                         builder == CUKD_BUILDER_BITONIC);
+                         builder == CUKD_BUILDER_ID_BITONIC);
   return build_tree_impl(d_points, n, *layout, d_world_bounds, stream, mr);
+}
+
+int cukd_build_tree_host(void *h_points, int32_t n, const cukd_layout *layout,
+                         void *h_world_bounds)
+{
+  g_last_error[0] = 0;
+  if (int rc = check_layout(layout)) return rc;
+  if (n < 1) return CUKD_OK;
+  if (!h_points) { set_last_error("points is null"); return CUKD_ERR_INVALID; }
+  if (layout->dim_offset >= 0 && !h_world_bounds) {
+    set_last_error("cukd::builder_host: asked to build k-d tree over nodes with explicit "
+                   "dims, but no memory for world bounds provided");
+    return CUKD_ERR_NO_BOUNDS;
+  }
+  const size_t bytes = (size_t)n * (size_t)layout->stride;
+  const size_t box_bytes = (size_t)2 * layout->num_dims * 4;
+  uint8_t *d = nullptr;
+  hipError_t e = hipMalloc((void **)&d, bytes + 256 + box_bytes);
+  if (e != hipSuccess) {
+    set_last_error("hipMalloc(%zu) failed: %s", bytes + 256 + box_bytes, hipGetErrorString(e));
+    return CUKD_ERR_ALLOC;
+  }
+  void *d_box = h_world_bounds ? (void *)(d + ((bytes + 255) & ~(size_t)255)) : nullptr;
+  int rc = CUKD_OK;
+  e = hipMemcpy(d, h_points, bytes, hipMemcpyDefault);
+  if (e == hipSuccess) {
+    rc = cukd_build_tree(d, n, layout, CUKD_BUILDER_ID_DEFAULT, d_box, nullptr, nullptr);
+    if (rc == CUKD_OK) {
+      e = hipMemcpy(h_points, d, bytes, hipMemcpyDefault);
+      if (e == hipSuccess && d_box) e = hipMemcpy(h_world_bounds, d_box, box_bytes, hipMemcpyDefault);
+    }
+  }
+  (void)hipFree(d);
+  if (rc != CUKD_OK) return rc;
+  if (e != hipSuccess) {
+    set_last_error("host build copy failed: %s", hipGetErrorString(e));
+    return CUKD_ERR_HIP;
+  }
+  return CUKD_OK;
 }
 
 int cukd_compute_bounds(void *d_bounds, const void *d_points, int32_t n,
@@ -547,6 +590,7 @@ int cukd_spatial_build(cukd_spatial_tree *tree, const void *d_points, int32_t n,
     set_last_error("leaf threshold %d exceeds the 16-bit leaf count", leaf_threshold);
     return CUKD_ERR_INVALID;
   }
+  if (int rc = screen_nan(d_points, n, *layout, stream)) return rc;
   MemResource mr{mem};
   return spatial_build_impl(tree, d_points, n, *layout, leaf_threshold, stream, mr);
 }
@@ -595,6 +639,7 @@ static int spatial_query(int traversal, int list_kind, int k, const cukd_spatial
   a.ids = d_ids;
   a.dist2 = d_dist2;
   a.ret = d_ret;
+  a.stats = g_stats;
   const hipError_t e = launch_spatial_query(layout->num_dims, traversal, a, stream);
   if (e != hipSuccess) {
     set_last_error("spatial query launch failed: %s", hipGetErrorString(e));
@@ -721,13 +766,18 @@ int cukd_forest_reserve_queries(cukd_forest *forest, int32_t max_queries, hipStr
   g_last_error[0] = 0;
   if (!forest || max_queries < 0) { set_last_error("bad argument"); return CUKD_ERR_INVALID; }
   const size_t need = query_order_temp_bytes(max_queries);
-  if (need <= forest->order_ws_bytes) return CUKD_OK;
+  std::lock_guard<std::mutex> lk(forest->order_mu);
+  if (need <= forest->order_ws_bytes && stream == forest->order_stream) return CUKD_OK;
+  const size_t bytes = need > forest->order_ws_bytes ? need : forest->order_ws_bytes;
   MemResource mr{forest->has_mem ? &forest->mem : nullptr};
-  if (forest->order_ws) mr.release(forest->order_ws, stream);
+  // the old scratch was only ever used on the old stream: free it there,
+  // after those queries
+  if (forest->order_ws) mr.release(forest->order_ws, forest->order_stream);
   forest->order_ws = nullptr;
   forest->order_ws_bytes = 0;
-  if (mr.alloc(&forest->order_ws, need, stream) != 0) return CUKD_ERR_ALLOC;
-  forest->order_ws_bytes = need;
+  forest->order_stream = stream;
+  if (mr.alloc(&forest->order_ws, bytes, stream) != 0) return CUKD_ERR_ALLOC;
+  forest->order_ws_bytes = bytes;
   return CUKD_OK;
 }
 
@@ -736,7 +786,11 @@ int cukd_forest_destroy(cukd_forest *forest, hipStream_t stream)
   g_last_error[0] = 0;
   if (!forest) return CUKD_OK;
   MemResource mr{forest->has_mem ? &forest->mem : nullptr};
-  if (forest->order_ws) mr.release(forest->order_ws, stream);
+  if (forest->order_ws) {
+    // queries on the reserving stream may still be using it
+    if (forest->order_stream != stream) (void)hipStreamSynchronize(forest->order_stream);
+    mr.release(forest->order_ws, stream);
+  }
   mr.release(forest->ws, stream);
   delete forest;
   return CUKD_OK;
@@ -777,8 +831,17 @@ static int forest_query(int traversal, int list_kind, int k, const cukd_forest *
   a.tree_bytes = (uint64_t)forest->plan.ntotal * (uint64_t)L.stride;
   a.stats = g_stats;
   MemResource mr{forest->has_mem ? &forest->mem : nullptr};
-  return run_queries(a, L.num_dims, traversal, flags, mr, forest->order_ws,
-                     forest->order_ws_bytes, true, stream, "forest query");
+  // the reserved ordering scratch serves queries on its own stream only;
+  // sorted queries on any other stream get scratch of their own, so two
+  // streams never share one Morton order
+  cukd_forest *f = const_cast<cukd_forest *>(forest);
+  std::unique_lock<std::mutex> lk(f->order_mu);
+  const bool own = f->order_ws && stream == f->order_stream;
+  void *ws = own ? f->order_ws : nullptr;
+  const size_t ws_bytes = own ? f->order_ws_bytes : 0;
+  if (!own) lk.unlock();
+  return run_queries(a, L.num_dims, traversal, flags, mr, ws, ws_bytes, true, stream,
+                     "forest query");
 }
 
 int cukd_forest_fcp(int traversal, const cukd_forest *forest, const void *d_points,

@@ -557,6 +557,7 @@ struct SpatialArgs {
   int32_t *ids;
   float *dist2;
   float *ret;
+  unsigned long long *stats;  // node + leaf-point visits (fcp.h:294,327), nullable
 };
 hipError_t launch_spatial_query(int D, int method, const SpatialArgs &a, hipStream_t s);
 int spatial_build_impl(cukd_spatial_tree *tree, const void *d_points, int n, const cukd_layout &L,

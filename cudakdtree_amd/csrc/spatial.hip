@@ -490,7 +490,8 @@ __global__ __launch_bounds__(128) void k_spatial_query(const cukd_spatial_node *
                                                        const uint32_t *prim_ids, TreeView data,
                                                        SpatialBounds bounds, const float *queries,
                                                        int nq, float cutoff, int k,
-                                                       int32_t *ids, float *dist2, float *ret)
+                                                       int32_t *ids, float *dist2, float *ret,
+                                                       unsigned long long *stats)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nq) return;
@@ -503,7 +504,10 @@ __global__ __launch_bounds__(128) void k_spatial_query(const cukd_spatial_node *
   float lo[D], hi[D];
 #pragma unroll
   for (int d = 0; d < D; d++) { lo[d] = bounds.v[d]; hi[d] = bounds.v[D + d]; }
-  const bool ok = cukd::detail::spatial_traverse<D, CCT>(res, q, nodes, prim_ids, lo, hi, data);
+  unsigned visits = 0;
+  const bool ok =
+      cukd::detail::spatial_traverse<D, CCT>(res, q, nodes, prim_ids, lo, hi, data, visits);
+  if (stats) atomicAdd(stats, (unsigned long long)visits);
   ResultOps<R>::write(st, i, k, ids, dist2, ret, no_lists);
   // stack overflow: fcp reports no result, knn returns -1 (fcp.h:317-320,
   // knn.h:620-623 print "STACK OVERFLOW" and return -1)
@@ -520,7 +524,7 @@ static hipError_t launch_spatial_m(const SpatialArgs &a, hipStream_t s)
 #define CUKD_SPQ(R)                                                                          \
   hipLaunchKernelGGL((k_spatial_query<D, CCT, R>), dim3(blocks), dim3(nt), 0, s, a.nodes,   \
                      a.prim_ids, a.data, a.bounds, a.queries, a.nq, a.cutoff, a.k, a.ids,   \
-                     a.dist2, a.ret)
+                     a.dist2, a.ret, a.stats)
   if (a.list_kind < 0) CUKD_SPQ(cukd::FCPResult);
   else if (a.list_kind == CUKD_LIST_HEAP) CUKD_SPQ(cukd::FlexHeapCandidateList);
   else if (a.k == 4) CUKD_SPQ(cukd::FixedCandidateList<4>);

@@ -13,9 +13,7 @@
 #include <math.h>
 #include <stdint.h>
 
-#ifndef __both__
-#define __both__ __host__ __device__
-#endif
+#include "cukd/common.h"
 
 namespace cukd {
 

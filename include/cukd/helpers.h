@@ -125,7 +125,7 @@ inline cukd_layout layout_of()
     unsigned char before[sizeof(data_t)];
     memcpy(before, raw, sizeof(raw));
     pt::set_coord(const_cast<point_t &>(p), d, (scalar_t)1);
-    if (memcmp(before, raw, sizeof(raw)) == 0 || raw[off + 4 * d] == 0 && raw[off + 4 * d + 3] == 0)
+    if (memcmp(before, raw, sizeof(raw)) == 0 || (raw[off + 4 * d] == 0 && raw[off + 4 * d + 3] == 0))
       throw std::runtime_error("cukd: point coordinates must be contiguous 4-byte scalars");
   }
   if (data_traits::has_explicit_dim) {

@@ -52,7 +52,7 @@ namespace detail {
 template <int D, bool CCT, class R, class NodeT, class Acc>
 inline __both__ bool spatial_traverse(R &result, const float (&q)[D], const NodeT *nodes,
                                       const uint32_t *prim_ids, const float (&lo)[D],
-                                      const float (&hi)[D], const Acc &data)
+                                      const float (&hi)[D], const Acc &data, unsigned &visits)
 {
   enum { DEPTH = 50 };
   float cull = result.initialCullDist2();
@@ -75,6 +75,7 @@ inline __both__ bool spatial_traverse(R &result, const float (&q)[D], const Node
   for (;;) {
     NodeT node;
     for (;;) {
+      visits++;
       node = nodes[node_id];
       // an empty root leaf (no points) has count 0 and offset ~0u
       if (node.count || node.offset == 0xffffffffu) break;
@@ -107,6 +108,7 @@ inline __both__ bool spatial_traverse(R &result, const float (&q)[D], const Node
     }
     for (int i = 0; i < node.count; i++) {
       const int prim = (int)prim_ids[node.offset + i];
+      visits++;
       float p[D];
       data.template point<D>(prim, p);
       cull = result.processCandidate(prim, sqr_dist<D>(p, q));
@@ -136,8 +138,12 @@ inline __both__ bool spatial_query(R &result, const SpatialKDTree<data_t, data_t
   to_floats(queryPoint, q);
   to_floats(tree.bounds.lower, lo);
   to_floats(tree.bounds.upper, hi);
-  return spatial_traverse<D, CCT>(result, q, tree.nodes, tree.primIDs, lo, hi,
-                                  TypedNodes<data_t, data_traits>{tree.data, tree.numPrims});
+  unsigned visits = 0;
+  const bool ok = spatial_traverse<D, CCT>(result, q, tree.nodes, tree.primIDs, lo, hi,
+                                           TypedNodes<data_t, data_traits>{tree.data, tree.numPrims},
+                                           visits);
+  count_visits(visits);
+  return ok;
 }
 }  // namespace detail
 

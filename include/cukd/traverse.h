@@ -253,6 +253,7 @@ inline __both__ void traverse_default(result_t &result, typename data_traits::po
   unsigned visits = 0;
   detail::traverse_stack_based<D>(result, q, detail::TypedNodes<data_t, data_traits>{d_nodes, numPoints},
                                   visits);
+  detail::count_visits(visits);
 }
 
 template <typename result_t, typename data_t, typename data_traits = default_data_traits<data_t>>
@@ -265,6 +266,7 @@ inline __both__ void traverse_stack_free(result_t &result, typename data_traits:
   unsigned visits = 0;
   detail::traverse_stack_free<D>(result, q, detail::TypedNodes<data_t, data_traits>{d_nodes, N}, eps,
                                  visits);
+  detail::count_visits(visits);
 }
 
 template <typename result_t, typename data_t, typename data_traits = default_data_traits<data_t>>
@@ -280,6 +282,7 @@ inline __both__ void traverse_cct(result_t &result, typename data_traits::point_
   unsigned visits = 0;
   detail::traverse_cct<D>(result, q, lo, hi, detail::TypedNodes<data_t, data_traits>{d_nodes, numPoints},
                           visits);
+  detail::count_visits(visits);
 }
 
 }  // namespace cukd

@@ -9,7 +9,8 @@
  * Conventions
  *  - All pointers named d_* are device pointers.  Every call is
  *    stream-ordered on 'stream' and returns before the GPU work is done,
- *    except where noted; no call synchronises the device or the stream.
+ *    except where noted (the NaN screen of cukd_build_tree, the host
+ *    builder, spatial builds, forest set-up and tuning runs wait).
  *  - Return value: CUKD_OK (0) or a negative CUKD_ERR_* code; the message
  *    of the last error on the calling thread is in cukd_last_error().
  *    Nothing throws across this boundary.
@@ -63,10 +64,10 @@ typedef struct cukd_mem_resource {
 } cukd_mem_resource;
 
 /* builder selection (builder.h:108-151) */
-#define CUKD_BUILDER_DEFAULT 0 /* cukd::buildTree                        */
-#define CUKD_BUILDER_THRUST 1  /* cukd::buildTree_thrust                 */
-#define CUKD_BUILDER_BITONIC 2 /* cukd::buildTree_bitonic                */
-#define CUKD_BUILDER_INPLACE 3 /* cukd::buildTree_inPlace                */
+#define CUKD_BUILDER_ID_DEFAULT 0 /* cukd::buildTree                        */
+#define CUKD_BUILDER_ID_THRUST 1  /* cukd::buildTree_thrust                 */
+#define CUKD_BUILDER_ID_BITONIC 2 /* cukd::buildTree_bitonic                */
+#define CUKD_BUILDER_ID_INPLACE 3 /* cukd::buildTree_inPlace                */
 
 /* traversal selection (fcp.h / knn.h namespaces) */
 #define CUKD_STACK_BASED 0 /* cukd::stackBased:: (traverse-default-stack-based.h) */
@@ -109,21 +110,37 @@ typedef struct cukd_mem_resource {
  * explicit-dim layouts (builder_thrust.h:469-473).  n < 1 returns CUKD_OK
  * without work (builder_thrust.h:441).
  *
- * CUKD_BUILDER_DEFAULT / _THRUST: radix-select builder, about 2.3n words
+ * CUKD_BUILDER_ID_DEFAULT / _THRUST: radix-select builder, about 2.3n words
  *   of temporary memory; the array is identical to buildTree_host's
  *   (builder_thrust.h:514-601) for every input without NaN coordinates,
  *   ties included.
- * CUKD_BUILDER_BITONIC: sorts each subtree inside the final array with a
+ * CUKD_BUILDER_ID_BITONIC: sorts each subtree inside the final array with a
  *   bitonic network; temporary memory n ints (builder.h:34-38), identical
  *   to buildTree_host's like the default builder.
- * CUKD_BUILDER_INPLACE: same network, no temporary memory at all
+ * CUKD_BUILDER_ID_INPLACE: same network, no temporary memory at all
  *   (builder.h:41-45); ties on a split coordinate are broken by the
  *   element's bytes, so the array equals buildTree_host's whenever the
  *   split coordinates decide (always on tie-free inputs) and is otherwise
- *   a valid, deterministic left-balanced tree. */
+ *   a valid, deterministic left-balanced tree.
+ *
+ * NaN coordinates: the reference compares coordinates with '<'
+ * (builder_thrust.h:401-404), which orders nothing once a NaN is present.
+ * For float layouts every builder first screens the coordinates (one read
+ * of the points and one wait for 'stream', as the reference's thrust
+ * builder waits once per level, builder_thrust.h:500) and returns CUKD_ERR_INVALID,
+ * with the array unchanged, if any is NaN.  Under HIP stream capture the
+ * screen is skipped, so the call stays capturable and does not wait. */
 int cukd_build_tree(void *d_points, int32_t n, const cukd_layout *layout,
                     int builder, void *d_world_bounds, hipStream_t stream,
                     const cukd_mem_resource *mem);
+
+/* buildTree_host (builder_thrust.h:114-118, 514-601): the same tree over
+ * points in host (or managed) memory, h_world_bounds (host, nullable unless
+ * explicit dims) receiving the bounds.  The points go to the GPU, are built
+ * by the default builder and come back; the call returns when the array is
+ * final.  Errors as cukd_build_tree. */
+int cukd_build_tree_host(void *h_points, int32_t n, const cukd_layout *layout,
+                         void *h_world_bounds);
 
 /* Bytes of temporary device memory cukd_build_tree will request. */
 int cukd_build_tree_temp_bytes(int32_t n, const cukd_layout *layout,
@@ -284,8 +301,12 @@ int cukd_forest_knn(int traversal, int list_kind, int k,
                     float *d_max_radius2, int flags, hipStream_t stream);
 
 /* Reserves the Morton-ordering scratch for forest queries of up to
- * max_queries queries, so that CUKD_QUERY_SORT forest queries allocate
- * nothing (e.g. inside a captured HIP graph). */
+ * max_queries queries on 'stream', so that CUKD_QUERY_SORT forest queries
+ * on that stream allocate nothing (e.g. inside a captured HIP graph).  The
+ * scratch serves that one stream: sorted forest queries on any other
+ * stream allocate scratch of their own (through the forest's memory
+ * resource), so queries on several streams never share it.  Reserving
+ * again from another stream moves the scratch to that stream. */
 int cukd_forest_reserve_queries(cukd_forest *forest, int32_t max_queries,
                                 hipStream_t stream);
 
@@ -293,16 +314,21 @@ int cukd_forest_reserve_queries(cukd_forest *forest, int32_t max_queries,
  *
  * No counterpart in the reference.  Requests (one tree, nq host queries,
  * FCP or kNN-k) are submitted from any thread; a worker thread packs
- * whatever is queued into one batch (admitted in arrival order while the
- * batch fits a quarter of budget_bytes of device memory, at most
- * max_requests_per_batch requests, <= 0 for no limit) and launches one
- * forest query per (k, cutoff) class.  Up to four batches are in flight,
- * each on a stream of its own, and each is completed (results copied back,
- * requests completed) as soon as it finishes, so a batch slowed by a few
- * long traversals does not delay the batches launched after it.
- * max_requests_per_batch = 1 serves each request with its own launches.
- * Results equal cukd_fcp / cukd_knn (fixed list) on the request's tree
- * alone. */
+ * whatever is queued into one batch slot (admitted in arrival order while
+ * the batch, laid out exactly as it will be launched, fits one eighth of
+ * budget_bytes of device memory, at most max_requests_per_batch requests,
+ * <= 0 for no limit) and launches one forest query per (k, cutoff) class.
+ * A heavy request (nq * max(k, 1) >= 65536) is batched alone, so light
+ * requests never wait for it inside a batch.  A request too large for one
+ * slot is queued in pieces that each fit one and completes with its last
+ * piece.  Up to eight batches are in flight, each on a stream of its own,
+ * and each is completed (results copied back, requests completed) as soon
+ * as it finishes, so a batch slowed by a few long traversals does not
+ * delay the batches launched after it.  max_requests_per_batch = 1 serves
+ * each request with its own launches.  Results equal cukd_fcp / cukd_knn
+ * (fixed list) on the request's tree alone.  A failed batch fails its own
+ * requests (their completion status); only a failed HIP call makes later
+ * polls report an error. */
 typedef struct cukd_server cukd_server;
 
 typedef struct cukd_completion {
@@ -321,7 +347,7 @@ typedef struct cukd_server_stats {
 } cukd_server_stats;
 
 /* d_points / d_world_bounds: the built forest's arrays (bounds required
- * for CUKD_CCT).  Allocates four device batch buffers and their pinned host
+ * for CUKD_CCT).  Allocates eight device batch buffers and their pinned host
  * mirrors (budget_bytes in total), creates the batches' streams, which
  * start after the work queued on 'stream' so far (the forest build, say),
  * and starts the worker thread. */

@@ -31,7 +31,7 @@ def test_python_constants_match_header_defines():
             assert getattr(ck, name) == val, name
             checked += 1
     for name in ("QUERY_SORT", "QUERY_AUTOTUNE", "QUERY_ONE_PER_LANE", "STACK_BASED", "CCT",
-                 "LIST_HEAP", "BUILDER_INPLACE", "ERR_UNSUPPORTED"):
+                 "LIST_HEAP", "BUILDER_ID_INPLACE", "ERR_UNSUPPORTED"):
         assert name in defs and hasattr(ck, name), name
     assert checked >= 20
 

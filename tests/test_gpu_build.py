@@ -190,17 +190,110 @@ def test_ten_million_builds_repeat_and_agree():
         assert torch.equal(t, ref), "default build %d differs from the bitonic build" % i
 
 
-@pytest.mark.slow
-def test_ten_million_properties():
-    """Full bench size: a valid k-d tree and a permutation of the input
-    (the oracle's 10M build is minutes on one core; the 2M case above
-    pins exact parity on the same code path)."""
+def _is_permutation(a, b):
+    a = np.ascontiguousarray(a).view(np.uint32).reshape(len(a), -1)
+    b = np.ascontiguousarray(b).view(np.uint32).reshape(len(b), -1)
+    return np.array_equal(a[np.lexsort(a.T)], b[np.lexsort(b.T)])
+
+
+@pytest.mark.parametrize("dist", ["uniform", "gmm"])
+def test_ten_million_default_equals_oracle(dist):
+    """The bench's tree size (10M float3; uniform 24-bit coordinates have
+    millions of exactly tied values per axis, the 64-centre GMM is the
+    clustered case): the default builder's array is byte-identical to the
+    oracle's buildTree_host restatement (builder_thrust.h:514-601)."""
+    n = 10_000_000
+    coords = datagen.uniform(n, 3, 1) if dist == "uniform" else datagen.gmm(n, 3, 11)
+    lay = ck.float_points(3)
+    print("10M %s: GPU build" % dist, flush=True)
+    got, gbox = gpu_build(coords, lay, want_bounds=True)
+    print("10M %s: oracle build" % dist, flush=True)
+    ref, rbox = oc.build_host(coords, lay, want_bounds=True)
+    if not np.array_equal(got, ref):
+        bad = np.nonzero(np.any(got != ref, axis=1))[0]
+        raise AssertionError("%d of %d nodes differ, first at %d" % (len(bad), n, bad[0]))
+    assert np.array_equal(gbox.view(rbox.dtype), rbox)
+
+
+def test_ten_million_inplace_valid_permutation():
+    """The in-place builder at 10M (ties included): a valid k-d tree over a
+    permutation of the input, and the same array on a second run."""
     n = 10_000_000
     coords = datagen.uniform(n, 3, 1)
     lay = ck.float_points(3)
-    got, _ = gpu_build(coords, lay)
+    print("10M in-place builds", flush=True)
+    got, _ = gpu_build(coords, lay, ck.BUILDER_INPLACE)
+    again, _ = gpu_build(coords, lay, ck.BUILDER_INPLACE)
+    assert np.array_equal(got, again)
     lv = np.floor(np.log2(np.arange(n, dtype=np.float64) + 1)).astype(np.int64) % 3
     check_kdtree_valid(got, lv, n)
-    a = coords.view(np.uint32)
-    b = got.view(np.uint32)
-    assert np.array_equal(a[np.lexsort(a.T)], b[np.lexsort(b.T)])
+    assert _is_permutation(coords, got)
+
+
+@pytest.mark.parametrize("kind", ["explicit", "photon"])
+@pytest.mark.parametrize("n", [150_000, 300_000, 1_000_000])
+def test_explicit_dims_large(kind, n):
+    """Explicit-dim trees large enough that the finish runs on subtrees of
+    1024..4096 elements (one tree: fewer than 512 finish workgroups), equal
+    to the oracle."""
+    coords = distribution("uniform" if n != 300_000 else "gmm", n, 3, seed=n)
+    lay, got = compare(coords, kind)
+    check_kdtree_valid(coords_of(got, lay), dims_of(got, lay), n)
+
+
+def test_explicit_dims_ten_million():
+    """PointPlusPayload {float3; int payload; int dim} (20 bytes) at 10M:
+    the default builder equals the bitonic builder (which is exact against
+    the oracle at every size tested), and the tree is valid with every
+    payload once."""
+    n = 10_000_000
+    coords = datagen.uniform(n, 3, 21)
+    lay = ck.point_plus_payload(3, explicit_dim=True)
+    elems = make_elements(coords, lay)
+    elems[:, 3] = np.arange(n, dtype=np.uint32)
+    print("10M explicit-dim builds", flush=True)
+    a, abox = gpu_build(elems, lay, ck.BUILDER_DEFAULT)
+    b, bbox = gpu_build(elems, lay, ck.BUILDER_BITONIC)
+    assert np.array_equal(a, b)
+    assert np.array_equal(abox, bbox)
+    check_kdtree_valid(coords_of(a, lay), dims_of(a, lay), n)
+    assert np.array_equal(np.sort(a[:, 3]), np.arange(n, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("builder", BUILDERS)
+@pytest.mark.parametrize("kind", ["float", "explicit"])
+def test_nan_rejected_array_unchanged(builder, kind):
+    """A NaN coordinate has no place in the reference's '<' order
+    (builder_thrust.h:401-404): every builder refuses it with
+    CUKD_ERR_INVALID and leaves the array as it was."""
+    coords = datagen.uniform(50_000, 3, 9)
+    coords[31_337, 1] = np.nan
+    lay = layout_for(kind, 3)
+    elems = make_elements(coords, lay)
+    t = torch.from_numpy(elems.copy()).cuda()
+    b = ck.bounds_tensor(lay)
+    with pytest.raises(ck.CukdError) as ei:
+        ck.build_tree(t, lay, builder, b)
+    assert ei.value.code == ck.ERR_INVALID and "NaN" in str(ei.value)
+    torch.cuda.synchronize()
+    assert np.array_equal(t.cpu().numpy(), elems)
+    with pytest.raises(ck.CukdError):
+        ck.buildTree_host(elems.copy(), lay, want_bounds=True)
+
+
+@pytest.mark.parametrize("kind", ["float", "int", "explicit", "photon"])
+def test_build_tree_host_equals_oracle(kind):
+    """buildTree_host (builder_thrust.h:114-118) over host memory: the
+    array and the host bounds box equal the oracle's."""
+    coords = distribution("ties", 77_777, 3, seed=12, scalar_float=kind != "int")
+    lay = layout_for(kind, 3)
+    elems = make_elements(coords, lay)
+    ref, rbox = oc.build_host(elems, lay, want_bounds=True)
+    got = elems.copy()
+    box = ck.buildTree_host(got, lay, want_bounds=True)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(box.view(np.uint32), rbox.view(np.uint32))
+    if kind == "explicit":
+        with pytest.raises(ck.CukdError) as ei:
+            ck.buildTree_host(elems.copy(), lay)
+        assert ei.value.code == ck.ERR_NO_BOUNDS

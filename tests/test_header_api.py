@@ -9,10 +9,54 @@ import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 BIN = os.path.join(HERE, "cpp", "header_api_test.bin")
+# tests/cpp/ref_callers_test.cpp built per compile-time builder choice
+# (reference builder.h:124-150)
+REF_CALLERS = {v: os.path.join(HERE, "cpp", "ref_callers_%s.bin" % v)
+               for v in ("thrust", "bitonic", "inplace")}
 
 
 def test_header_api_program_built():
     assert os.path.exists(BIN), "run __graft_entry__.build() (make -C tests/cpp)"
+    for path in REF_CALLERS.values():
+        assert os.path.exists(path), path
+
+
+def test_builder_macros_select_at_compile_time():
+    """The ABI's builder ids do not take the reference's macro names, so
+    -DCUKD_BUILDER_INPLACE / _BITONIC reach cukd::buildTree: the header
+    compiles without a redefinition warning and dispatches on the macro."""
+    import subprocess as sp
+    root = os.path.dirname(HERE)
+    src = ('#include "cukd/builder.h"\n'
+           'template void cukd::buildTree<float3>(float3*, int, cukd::box_t<float3>*, '
+           'hipStream_t, cukd::GpuMemoryResource&);\n')
+    for macro in ("", "-DCUKD_BUILDER_INPLACE", "-DCUKD_BUILDER_BITONIC"):
+        cmd = ["/opt/rocm/bin/hipcc", "-x", "hip", "--offload-arch=gfx950", "-std=c++17",
+               "-fsyntax-only", "-Werror", "-Wno-unused-command-line-argument",
+               "-I", os.path.join(root, "include"), "-"]
+        if macro:
+            cmd.insert(1, macro)
+        r = sp.run(cmd, input=src, capture_output=True, text=True)
+        assert r.returncode == 0, (macro, r.stderr)
+    hdr = open(os.path.join(root, "include", "cukd", "builder.h")).read()
+    assert "#if defined(CUKD_BUILDER_INPLACE)" in hdr
+    assert "#elif defined(CUKD_BUILDER_BITONIC)" in hdr
+    abi = open(os.path.join(root, "include", "cukd_amd.h")).read()
+    for name in ("CUKD_BUILDER_INPLACE", "CUKD_BUILDER_BITONIC", "CUKD_BUILDER_THRUST"):
+        assert "#define %s " % name not in abi, name
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", ["thrust", "bitonic", "inplace"])
+def test_reference_callers_program_runs(variant):
+    """buildTree_host == buildTree_thrust; the compile-time builder choice
+    seen through the memory resource (in-place: no allocation); CUKD_STATS
+    counts in a user kernel equal the library's; NaN input rejected."""
+    print("ref_callers", variant, flush=True)
+    r = subprocess.run([REF_CALLERS[variant]], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("OK")
 
 
 @pytest.mark.gpu
