@@ -47,7 +47,7 @@ QUERY_PRESORTED = 8
 EXPORTED_SYMBOLS = (
     "cukd_build_tree", "cukd_build_tree_host", "cukd_build_tree_temp_bytes",
     "cukd_compute_bounds",
-    "cukd_fcp", "cukd_knn", "cukd_spatial_build", "cukd_spatial_free",
+    "cukd_fcp", "cukd_knn", "cukd_knn_noalloc", "cukd_spatial_build", "cukd_spatial_free",
     "cukd_spatial_fcp", "cukd_spatial_knn", "cukd_set_traversal_stats",
     "cukd_knn_accumulate", "cukd_knn_extract", "cukd_query_order",
     "cukd_forest_create", "cukd_forest_workspace_bytes", "cukd_forest_build",
@@ -152,6 +152,8 @@ def lib():
                            ctypes.c_int, P, P]
     L.cukd_knn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, i32, LP, P, P,
                            i32, f32, P, P, P, ctypes.c_int, P, P]
+    L.cukd_knn_noalloc.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, i32, LP, P, P,
+                                   i32, f32, P, P, P, ctypes.c_int, P]
     SP = ctypes.POINTER(SpatialTree)
     L.cukd_spatial_build.argtypes = [SP, P, i32, LP, ctypes.c_int, P, P]
     L.cukd_spatial_free.argtypes = [SP, P, P]

@@ -463,12 +463,17 @@ int cukd_fcp(int traversal, const void *d_tree, int32_t n, const cukd_layout *la
   return run_queries(a, layout->num_dims, traversal, flags, mr, nullptr, 0, true, stream, "fcp");
 }
 
-int cukd_knn(int traversal, int list_kind, int k, const void *d_tree, int32_t n,
-             const cukd_layout *layout, const void *d_world_bounds, const float *d_queries,
-             int32_t nq, float cutoff, int32_t *d_ids, float *d_dist2, float *d_max_radius2,
-             int flags, hipStream_t stream, const cukd_mem_resource *mem)
+}  // extern "C"
+
+// cukd_knn with the ordering scratch given: order_ws (order_ws_bytes) when
+// non-null, else allocated through mr.  tune = false only reads the
+// autotuner's cache.
+static int knn_entry(int traversal, int list_kind, int k, const void *d_tree, int32_t n,
+                     const cukd_layout *layout, const void *d_world_bounds,
+                     const float *d_queries, int32_t nq, float cutoff, int32_t *d_ids,
+                     float *d_dist2, float *d_max_radius2, int flags, hipStream_t stream,
+                     const MemResource &mr, void *order_ws, size_t order_ws_bytes, bool tune)
 {
-  g_last_error[0] = 0;
   if (int rc = check_query_common(traversal, d_tree, n, layout, d_world_bounds, d_queries,
                                   nq, d_ids))
     return rc;
@@ -479,7 +484,6 @@ int cukd_knn(int traversal, int list_kind, int k, const void *d_tree, int32_t n,
   }
   if (nq > 0 && !d_dist2) { set_last_error("d_dist2 is null"); return CUKD_ERR_INVALID; }
   if (nq == 0) return CUKD_OK;
-  MemResource mr{mem};
   QueryArgs a;
   a.t = make_view(d_tree, n, layout);
   a.queries = d_queries;
@@ -496,7 +500,90 @@ int cukd_knn(int traversal, int list_kind, int k, const void *d_tree, int32_t n,
   a.la = ListArgs{nullptr, 0, 0};
   a.fq = ForestQ{nullptr, nullptr, 0, nullptr};
   a.stats = g_stats;
-  return run_queries(a, layout->num_dims, traversal, flags, mr, nullptr, 0, true, stream, "knn");
+  return run_queries(a, layout->num_dims, traversal, flags, mr, order_ws, order_ws_bytes, tune,
+                     stream, "knn");
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free kNN (cukd_knn_noalloc): the Morton-ordering scratch is a
+// file-scope device array, one copy per GPU, held until the process ends
+// ---------------------------------------------------------------------------
+
+enum : int32_t { NOALLOC_MAX_QUERIES = 1 << 22 };
+static constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)52 << 20;
+
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::map<int, uint8_t *> g_scratch_ptr;  // device -> its copy of 'scratch'
+
+// The current device's address of 'scratch', looked up once per device.
+// (The array's name in host code is only a stand-in for the symbol.)
+static int scratch_ptr(uint8_t **p)
+{
+  int dev = 0;
+  CUKD_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    if (query_order_temp_bytes(NOALLOC_MAX_QUERIES) > NOALLOC_SCRATCH_BYTES) {
+      set_last_error("internal: the static scratch is smaller than %d queries need",
+                     NOALLOC_MAX_QUERIES);
+      return CUKD_ERR_INVALID;
+    }
+    void *a = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !a) {
+      set_last_error("hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return CUKD_ERR_HIP;
+    }
+    it = g_scratch_ptr.emplace(dev, (uint8_t *)a).first;
+  }
+  *p = it->second;
+  return CUKD_OK;
+}
+
+// a memory resource that refuses every request: no path of
+// cukd_knn_noalloc may allocate, and one that tried would fail loudly
+static int refuse_malloc(void *, void **ptr, size_t, hipStream_t)
+{
+  *ptr = nullptr;
+  return 1;
+}
+static int refuse_free(void *, void *, hipStream_t) { return 0; }
+
+extern "C" {
+
+int cukd_knn_noalloc(int traversal, int list_kind, int k, const void *d_tree, int32_t n,
+                     const cukd_layout *layout, const void *d_world_bounds,
+                     const float *d_queries, int32_t nq, float cutoff, int32_t *d_ids,
+                     float *d_dist2, float *d_max_radius2, int flags, hipStream_t stream)
+{
+  g_last_error[0] = 0;
+  if (nq > NOALLOC_MAX_QUERIES) {
+    set_last_error("nq = %d exceeds the %d queries that cukd_knn_noalloc's static scratch "
+                   "holds",
+                   nq, NOALLOC_MAX_QUERIES);
+    return CUKD_ERR_INVALID;
+  }
+  uint8_t *ws = nullptr;
+  if (int rc = scratch_ptr(&ws)) return rc;
+  static const cukd_mem_resource refuse = {nullptr, refuse_malloc, refuse_free};
+  MemResource mr{&refuse};
+  return knn_entry(traversal, list_kind, k, d_tree, n, layout, d_world_bounds, d_queries, nq,
+                   cutoff, d_ids, d_dist2, d_max_radius2, flags, stream, mr, ws,
+                   NOALLOC_SCRATCH_BYTES, false);
+}
+
+int cukd_knn(int traversal, int list_kind, int k, const void *d_tree, int32_t n,
+             const cukd_layout *layout, const void *d_world_bounds, const float *d_queries,
+             int32_t nq, float cutoff, int32_t *d_ids, float *d_dist2, float *d_max_radius2,
+             int flags, hipStream_t stream, const cukd_mem_resource *mem)
+{
+  g_last_error[0] = 0;
+  MemResource mr{mem};
+  return knn_entry(traversal, list_kind, k, d_tree, n, layout, d_world_bounds, d_queries, nq,
+                   cutoff, d_ids, d_dist2, d_max_radius2, flags, stream, mr, nullptr, 0, true);
 }
 
 int cukd_knn_accumulate(int traversal, int k, const void *d_tree, int32_t n,

@@ -176,6 +176,23 @@ int cukd_knn(int traversal, int list_kind, int k, const void *d_tree,
              float *d_max_radius2, int flags, hipStream_t stream,
              const cukd_mem_resource *mem);
 
+/* cukd_knn without any allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as cukd_knn, less the memory resource.
+ * CUKD_QUERY_SORT orders the queries in a library-owned device array of
+ * 52 MiB, one per GPU, allocated when the library's code is loaded and held
+ * until the process ends, so nq is limited to 4194304 (2^22; larger is
+ * CUKD_ERR_INVALID).  CUKD_QUERY_AUTOTUNE only uses launch shapes an
+ * earlier cukd_knn call has tuned; it never tunes.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (several streams or
+ * threads at once) race on it and must not be made. */
+int cukd_knn_noalloc(int traversal, int list_kind, int k, const void *d_tree,
+                     int32_t n, const cukd_layout *layout,
+                     const void *d_world_bounds, const float *d_queries,
+                     int32_t nq, float cutoff, int32_t *d_ids, float *d_dist2,
+                     float *d_max_radius2, int flags, hipStream_t stream);
+
 /* The launch order CUKD_QUERY_SORT uses: d_order (nq int32) receives a
  * permutation of 0..nq-1 listing the queries by the Morton code of their
  * coordinates (18-bit codes over the batch's bounding box; the order of
