@@ -7,12 +7,20 @@ Workload (the 10M-point float3 case of the builder/query scaling config):
     of 1M uniform float3 queries (seed 2) over the resident tree.
 
 Prints one JSON line.  value = queries/s of the kNN step; the build time,
-FCP rate and the other headline figures ride along under "metrics".
-Inputs are resident in HBM before any timed region starts.
+FCP rate and the other headline figures ride along under "metrics", the
+small plain-method case and the spatial k-d tree case under "metrics.c1"
+and "metrics.spatial", the forest and serving forms under
+"metrics.forest" and "metrics.serve".  Inputs are resident in HBM before
+any timed region starts.  "parity" records the checks this run made
+against the oracle (test infrastructure, never the measured path): the
+10M tree byte for byte, and the headline launch's results on a 100K-query
+sample.
 
   python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
 """
 import argparse
+import ctypes
+import hashlib
 import json
 import math
 import os
@@ -23,6 +31,33 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+# source files that define the headline kNN kernel; the committed counter
+# summary records their hash, and its traffic figure is used only while
+# the hash still matches
+KNN_KERNEL_SOURCES = ("cudakdtree_amd/csrc/query_impl.h", "cudakdtree_amd/csrc/query_inst.hip",
+                      "cudakdtree_amd/csrc/cukd_common.h", "include/cukd/traverse.h",
+                      "include/cukd/knn.h", "include/cukd/fcp.h", "cudakdtree_amd/build.py")
+
+
+def kernel_source_sha():
+    h = hashlib.sha256()
+    for rel in KNN_KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(rel.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
+def host_cores():
+    """(threads used for the CPU baseline, nproc, CPUs this process may run
+    on).  A one-GPU job's share of the GPU box's host is 16 cores; the
+    baseline uses that share (or fewer if the affinity mask is smaller)."""
+    nproc = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = nproc
+    return max(1, min(16, allowed)), nproc, allowed
 
 
 def _pct(a, p):
@@ -62,30 +97,30 @@ def forest_bench(args, ck, datagen, torch, np):
     print("[bench] forest: %d trees, %d points, %d queries" % (T, N, NQ), file=sys.stderr,
           flush=True)
 
-    # plain: one build + one kNN launch sequence per tree
+    # plain: one cukd_build_tree + one cukd_knn per tree, the loop in C++
+    # (cudakdtree_amd/cli/plain_forest.cpp), autotuned launch shapes
     p_pts = torch.empty_like(orig)
     p_ids = torch.empty((NQ, k), dtype=torch.int32, device=dev)
     p_d2 = torch.empty((NQ, k), dtype=torch.float32, device=dev)
     p_r = torch.empty(NQ, dtype=torch.float32, device=dev)
+    plain_lib = ctypes.CDLL(os.path.join(ROOT, "cudakdtree_amd", "cli", "libcukd_plain.so"))
+    ms = ctypes.c_double(0.0)
+    h_off = np.ascontiguousarray(off, dtype=np.int64)
+    h_qoff = np.ascontiguousarray(qoff, dtype=np.int64)
+    cs = torch.cuda.current_stream()
+    rc = plain_lib.cukd_plain_forest_run(
+        ctypes.c_void_p(orig.data_ptr()), ctypes.c_void_p(p_pts.data_ptr()),
+        h_off.ctypes.data_as(ctypes.c_void_p), ctypes.c_int32(T),
+        ctypes.c_void_p(queries.data_ptr()), h_qoff.ctypes.data_as(ctypes.c_void_p),
+        ctypes.c_int32(k), ctypes.c_void_p(p_ids.data_ptr()), ctypes.c_void_p(p_d2.data_ptr()),
+        ctypes.c_void_p(p_r.data_ptr()), ctypes.c_int32(ck.QUERY_AUTOTUNE),
+        ctypes.c_void_p(cs.cuda_stream), ctypes.c_int32(args.forest_steps), ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError("plain forest run failed: %s" % ck.lib().cukd_last_error().decode())
+    plain_ms = ms.value
 
-    def plain():
-        p_pts.copy_(orig)
-        for t in range(T):
-            a, b, qa, qb = int(off[t]), int(off[t + 1]), int(qoff[t]), int(qoff[t + 1])
-            ck.build_tree(p_pts[a:b], lay)
-            if qb > qa:
-                ck.knn(p_pts[a:b], lay, queries[qa:qb], k,
-                       out=(p_ids[qa:qb], p_d2[qa:qb], p_r[qa:qb]))
-
-    plain()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.forest_steps):
-        plain()
-    torch.cuda.synchronize()
-    plain_ms = (time.perf_counter() - t0) * 1e3 / args.forest_steps
-
-    # batched: forest build + one kNN launch, eager and as a graph
+    # batched: forest build + one kNN launch, eager and as a graph; the kNN
+    # launch shape is autotuned on the first (eager) call, before capture
     f = ck.Forest(off, lay)
     b_pts = torch.empty_like(orig)
     b_ids = torch.empty_like(p_ids)
@@ -95,7 +130,8 @@ def forest_bench(args, ck, datagen, torch, np):
     def batched(stream=None):
         b_pts.copy_(orig)
         f.build(b_pts, stream=stream)
-        f.knn(b_pts, queries, k, seg_off, seg_tree, stream=stream, out=(b_ids, b_d2, b_r))
+        f.knn(b_pts, queries, k, seg_off, seg_tree, stream=stream, out=(b_ids, b_d2, b_r),
+              flags=ck.QUERY_AUTOTUNE)
 
     batched()
     torch.cuda.synchronize()
@@ -119,6 +155,8 @@ def forest_bench(args, ck, datagen, torch, np):
     print("[bench] forest build+knn: plain %.1f ms, batched eager %.1f ms, graph %.1f ms, "
           "identical: %s" % (plain_ms, eager_ms, graph_ms, same), file=sys.stderr, flush=True)
     out = {"forest_trees": T, "forest_points": N, "forest_queries": NQ, "forest_k": k,
+           "forest_plain": "one cukd_build_tree + one cukd_knn per tree, looped in C++ "
+                           "(cudakdtree_amd/cli/plain_forest.cpp); both legs autotuned",
            "forest_plain_ms": round(plain_ms, 3), "forest_batched_eager_ms": round(eager_ms, 3),
            "forest_batched_graph_ms": round(graph_ms, 3),
            "batched_vs_plain_speedup": round(plain_ms / graph_ms, 3),
@@ -260,6 +298,173 @@ def serve_bench(args, ck, datagen, torch, np):
     return out
 
 
+def c1_bench(args, ck, datagen, torch, np):
+    """The plain method at its small size: a left-balanced tree over 100K
+    float3 points uniform in [0,1)^3 (seed 1) built by buildTree, then FCP
+    (stack-based) for 100K uniform queries (seed 2).  GPU: event-timed
+    build (median of 10) and FCP launch; CPU: the oracle's buildTree_host
+    (one core) and its host FCP on the allowed cores."""
+    dev = torch.device("cuda:0")
+    lay = ck.float_points(3)
+    n = nq = 100_000
+    pts_h = datagen.uniform(n, 3, 1)
+    q_h = datagen.uniform(nq, 3, 2)
+    src = torch.from_numpy(pts_h).to(dev)
+    q = torch.from_numpy(q_h).to(dev)
+    t = torch.empty_like(src)
+    ids = torch.empty(nq, dtype=torch.int32, device=dev)
+    d2 = torch.empty(nq, dtype=torch.float32, device=dev)
+    s = torch.cuda.current_stream()
+    b_ms, f_ms = [], []
+    for i in range(12):
+        t.copy_(src)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e2 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        ck.buildTree(t, lay)
+        e1.record(s)
+        ck.fcp(t, lay, q, out=(ids, d2))
+        e2.record(s)
+        torch.cuda.synchronize()
+        if i >= 2:
+            b_ms.append(e0.elapsed_time(e1))
+            f_ms.append(e1.elapsed_time(e2))
+    out = {"workload": "100K float3 U[0,1)^3 buildTree + 100K FCP (stack-based)",
+           "gpu_build_ms": round(_pct(b_ms, 50), 4), "gpu_fcp_ms": round(_pct(f_ms, 50), 4),
+           "gpu_fcp_queries_per_s": round(nq / (_pct(f_ms, 50) / 1e3), 1)}
+    if not args.no_cpu:
+        import oracle as oc
+        threads = host_cores()[0]
+        t0 = time.perf_counter()
+        ref, _ = oc.build_host(pts_h, lay)
+        cb = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        rid, rd2 = oc.fcp(oc.STACK_BASED, ref, lay, q_h, threads=threads)
+        cf = (time.perf_counter() - t0) * 1e3
+        out.update({"cpu_build_ms_single_core": round(cb, 2), "cpu_fcp_ms": round(cf, 2),
+                    "cpu_fcp_cores": threads,
+                    "_tree_ok": bool(np.array_equal(ref, t.cpu().numpy())),
+                    "_fcp_ok": bool(np.array_equal(rid, ids.cpu().numpy()) and np.array_equal(
+                        rd2.view(np.uint32), d2.cpu().numpy().view(np.uint32)))})
+    print("[bench] c1 100K: build %.3f ms, fcp %.3f ms" % (out["gpu_build_ms"], out["gpu_fcp_ms"]),
+          file=sys.stderr, flush=True)
+    return out
+
+
+def spatial_bench(args, ck, datagen, torch, np):
+    """The spatial k-d tree case: 4M float4 points uniform in [-1,1)^4 with
+    1% exact duplicates; spatial build (the reference's host-synchronised
+    round loop), then kNN k=16 with cutoff 0.05 (stack-based,
+    FixedCandidateList<16>) for 2M queries, half data points plus
+    N(0, 0.001) jitter and half uniform; and a 2M int3 set uniform in
+    [0, 2^20) (1% duplicates), build only (the reference has no int
+    distance math).  CPU: the oracle's spatial build (one core) and its
+    kNN on a 50K-query sample (allowed cores)."""
+    dev = torch.device("cuda:0")
+    n, D, nq = 4_000_000, 4, 2_000_000
+    pts_h = datagen.uniform(n, D, 11, -1.0, 1.0)
+    dup = (datagen.splitmix64(12, n // 100) % np.uint64(n)).astype(np.int64)
+    pts_h[n - n // 100:] = pts_h[dup]
+    half = nq // 2
+    pick = (datagen.splitmix64(13, half) % np.uint64(n)).astype(np.int64)
+    jit = (datagen.gaussian(half * D, 14).reshape(half, D) * 0.001).astype(np.float32)
+    q_h = np.concatenate([pts_h[pick] + jit,
+                          datagen.uniform(nq - half, D, 15, -1.0, 1.0)]).astype(np.float32)
+    lay = ck.float_points(D)
+    pts = torch.from_numpy(pts_h).to(dev)
+    q = torch.from_numpy(q_h).to(dev)
+    torch.cuda.synchronize()
+    bms = []
+    tree = None
+    for i in range(4):
+        if tree is not None:
+            ck.free(tree)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tree = ck.buildSpatialTree(pts, lay)  # synchronises, as the reference's build does
+        bms.append((time.perf_counter() - t0) * 1e3)
+    k, r = 16, 0.05
+    ids = torch.empty((nq, k), dtype=torch.int32, device=dev)
+    d2 = torch.empty((nq, k), dtype=torch.float32, device=dev)
+    ret = torch.empty(nq, dtype=torch.float32, device=dev)
+    s = torch.cuda.current_stream()
+    qms = []
+    for i in range(6):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        lib = ck.lib()
+        rc = lib.cukd_spatial_knn(ck.STACK_BASED, ck.LIST_FIXED, k, ctypes.byref(tree.raw),
+                                  ctypes.byref(lay), ctypes.c_void_p(q.data_ptr()), nq, r,
+                                  ctypes.c_void_p(ids.data_ptr()), ctypes.c_void_p(d2.data_ptr()),
+                                  ctypes.c_void_p(ret.data_ptr()), ctypes.c_void_p(s.cuda_stream))
+        assert rc == 0, lib.cukd_last_error()
+        e1.record(s)
+        torch.cuda.synchronize()
+        if i >= 1:
+            qms.append(e0.elapsed_time(e1))
+    # int3 set, build only
+    ni = 2_000_000
+    ci = (datagen.splitmix64(17, ni * 3) >> np.uint64(44)).astype(np.int32).reshape(ni, 3)
+    src = (datagen.splitmix64(18, ni // 100) % np.uint64(ni)).astype(np.int64)
+    ci[ni - ni // 100:] = ci[src]
+    ilay = ck.int_points(3)
+    dci = torch.from_numpy(ci).to(dev)
+    ims = []
+    itree = None
+    for i in range(4):
+        if itree is not None:
+            ck.free(itree)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        itree = ck.buildSpatialTree(dci, ilay)
+        ims.append((time.perf_counter() - t0) * 1e3)
+    out = {"workload": "spatial k-d tree: 4M float4 U[-1,1)^4 (1% duplicates) build + kNN k=16 "
+                       "r=0.05 (stack-based, fixed list) of 2M queries; 2M int3 build",
+           "gpu_build_ms_4M_float4": round(min(bms[1:]), 3),
+           "gpu_knn16_ms_2M": round(_pct(qms, 50), 3),
+           "gpu_knn16_queries_per_s": round(nq / (_pct(qms, 50) / 1e3), 1),
+           "gpu_build_ms_2M_int3": round(min(ims[1:]), 3), "nodes_4M": tree.num_nodes}
+    if not args.no_cpu:
+        import oracle as oc
+        threads = host_cores()[0]
+        t0 = time.perf_counter()
+        rn, rp, rb = oc.spatial_build(pts_h, lay)
+        cb = (time.perf_counter() - t0) * 1e3
+        build_ok = bool(np.array_equal(tree.nodes_numpy().view(np.uint8), rn.view(np.uint8))
+                        and np.array_equal(tree.prims_numpy(), rp))
+        m = 50_000
+        sel = np.concatenate([np.arange(m // 2), half + np.arange(m - m // 2)])
+        t0 = time.perf_counter()
+        cid, cd2, cret = oc.spatial_knn(oc.STACK_BASED, oc.LIST_FIXED, k, rn, rp, rb, pts_h, lay,
+                                        q_h[sel], r, threads=threads)
+        cq = time.perf_counter() - t0
+        gi, gd = ids.cpu().numpy()[sel], d2.cpu().numpy()[sel]
+        knn_ok = bool(np.array_equal(cid, gi) and np.array_equal(cd2.view(np.uint32),
+                                                                 gd.view(np.uint32)))
+        t0 = time.perf_counter()
+        irn, irp, _ = oc.spatial_build(ci, ilay)
+        cib = (time.perf_counter() - t0) * 1e3
+        int_ok = bool(np.array_equal(itree.nodes_numpy().view(np.uint8), irn.view(np.uint8))
+                      and np.array_equal(itree.prims_numpy(), irp))
+        out.update({"cpu_build_ms_4M_float4_single_core": round(cb, 1),
+                    "cpu_knn16_queries_per_s": round(m / cq, 1), "cpu_knn16_cores": threads,
+                    "cpu_knn16_sample": "%d of the 2M queries (half from each half)" % m,
+                    "cpu_build_ms_2M_int3_single_core": round(cib, 1),
+                    "_spatial_build": build_ok, "_spatial_knn_sample": knn_ok,
+                    "_spatial_int3_build": int_ok})
+    ck.free(tree)
+    ck.free(itree)
+    print("[bench] spatial: 4M float4 build %.1f ms, knn16 %.3f ms for 2M queries, 2M int3 "
+          "build %.1f ms" % (out["gpu_build_ms_4M_float4"], out["gpu_knn16_ms_2M"],
+                            out["gpu_build_ms_2M_int3"]), file=sys.stderr, flush=True)
+    del pts, q, ids, d2, ret, dci
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,6 +487,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0,
                     help="Poisson request rate (requests/s) of the serving trace")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-spatial", action="store_true")
     ap.add_argument("--probe", action="store_true",
                     help="for hardware-counter runs: one 1 GiB device copy (the counters' "
                          "calibration), one build, one kNN and one FCP launch, no JSON line")
@@ -407,6 +614,10 @@ def main():
     ev_ms = e0.elapsed_time(e1) / args.steps
     ms_per_step = wall * 1e3 / args.steps
     qps = nq / (ms_per_step / 1e3)
+    # the headline launch's results on the CPU sample, for the parity check
+    head_ids = ids[: args.cpu_sample].cpu().numpy()
+    head_d2 = d2[: args.cpu_sample].cpu().numpy()
+    head_ret = ret[: args.cpu_sample].cpu().numpy()
 
     # node visits of this exact traversal (CUKD_STATS semantics)
     counter = torch.zeros(1, dtype=torch.int64, device=dev)
@@ -458,43 +669,71 @@ def main():
     torch.cuda.synchronize()
     fcp_ms = (time.perf_counter() - t0) * 1e3 / args.steps
 
-    # ---------------- batched forest vs plain per-tree launches ----------------
+    # ---------------- the other configurations' legs ----------------
+    c1 = None if args.no_c1 else c1_bench(args, ck, datagen, torch, np)
+    spatial = None if args.no_spatial else spatial_bench(args, ck, datagen, torch, np)
     forest = None if args.no_forest else forest_bench(args, ck, datagen, torch, np)
     serve = None if args.no_serve else serve_bench(args, ck, datagen, torch, np)
 
-    # ---------------- CPU baseline (oracle, host cores) ----------------
-    cpu = None
+    # ---------------- CPU baseline and parity (oracle, host cores) ----------------
+    cpu, parity = None, None
     if not args.no_cpu:
-        try:
-            import oracle as oc
-            threads = min(16, os.cpu_count() or 1)
-            tree_h = tree.cpu().numpy()
-            qs = q_h[: args.cpu_sample]
-            t0 = time.perf_counter()
-            oc.knn(oc.STACK_BASED, oc.LIST_FIXED, k, tree_h, lay, qs, threads=threads)
-            cpu_s = time.perf_counter() - t0
-            cpu = {"value": round(len(qs) / cpu_s, 1), "unit": "queries/s", "cores": threads,
-                   "kind": "port",
-                   "sample": "%d of the %d kNN(k=%d) queries over the same 10M-point tree, "
-                             "oracle stack-based traversal, OpenMP over queries"
-                             % (len(qs), nq, k)}
-            nb = 1_000_000
-            t0 = time.perf_counter()
-            oc.build_host(pts_h[:nb], lay)
-            cpu["build_ms_1M_single_core"] = round((time.perf_counter() - t0) * 1e3, 1)
-        except Exception as e:  # the baseline is reported, never required
-            cpu = {"value": None, "unit": "queries/s", "cores": 0, "kind": "port",
-                   "sample": "failed: %s" % e}
+        import oracle as oc
+        threads, nproc, allowed = host_cores()
+        tree_h = tree.cpu().numpy()
+        # the 10M host build, once (buildTree_host is single-threaded), and
+        # the GPU tree against it byte for byte
+        print("[bench] oracle: 10M host build (single core)", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        ref_tree, _ = oc.build_host(pts_h, lay)
+        host_build_ms = (time.perf_counter() - t0) * 1e3
+        tree_ok = bool(np.array_equal(ref_tree, tree_h))
+        del ref_tree
+        # the headline kNN on a sample of its queries, all allowed cores
+        qs = q_h[: args.cpu_sample]
+        t0 = time.perf_counter()
+        rid, rd2, rret = oc.knn(oc.STACK_BASED, oc.LIST_FIXED, k, tree_h, lay, qs,
+                                threads=threads)
+        cpu_s = time.perf_counter() - t0
+        knn_ok = bool(np.array_equal(rid, head_ids)
+                      and np.array_equal(rd2.view(np.uint32), head_d2.view(np.uint32))
+                      and np.array_equal(rret.view(np.uint32), head_ret.view(np.uint32)))
+        cpu = {"value": round(len(qs) / cpu_s, 1), "unit": "queries/s", "cores": threads,
+               "nproc": nproc, "affinity_cpus": allowed, "kind": "port",
+               "sample": "%d of the %d kNN(k=%d) queries over the same 10M-point tree, oracle "
+                         "stack-based traversal, OpenMP over queries on %d threads (the "
+                         "one-GPU job's share of the host)" % (len(qs), nq, k, threads),
+               "build_ms_10M_single_core": round(host_build_ms, 1),
+               "build_note": "oracle buildTree_host restatement at 10M, run once, one core"}
+        parity = {"tree_10M": tree_ok, "knn_100K": knn_ok}
+        if c1:
+            parity.update({"c1_tree": c1.pop("_tree_ok"), "c1_fcp": c1.pop("_fcp_ok")})
+        if spatial:
+            parity.update({k2: spatial.pop("_" + k2) for k2 in
+                           ("spatial_build", "spatial_knn_sample", "spatial_int3_build")})
+        print("[bench] parity: %s" % parity, file=sys.stderr, flush=True)
 
     # HBM bytes per kNN launch from the committed hardware-counter summary of
     # this same workload (FETCH_SIZE + WRITE_SIZE, each scaled by what the
-    # counter reads for a 1 GiB device copy)
-    traffic = None
+    # counter reads for a 1 GiB device copy), used only if it was measured
+    # on this kernel's current sources
+    traffic, traffic_note = None, "no counter summary"
     try:
         with open(os.path.join(ROOT, "profiles", "pmc_knn_10M_float3_k8.json")) as fh:
-            traffic = json.load(fh)["traffic_bytes_per_launch"]
+            pmc = json.load(fh)
+        if pmc.get("kernel_source_sha") == kernel_source_sha():
+            traffic = pmc["traffic_bytes_per_launch"]
+            traffic_note = "profiles/pmc_knn_10M_float3_k8.json, kernel sources %s" % \
+                pmc["kernel_source_sha"]
+        else:
+            traffic_note = "counter summary is of other kernel sources (%s, now %s)" % (
+                pmc.get("kernel_source_sha"), kernel_source_sha())
     except (OSError, ValueError, KeyError):
         pass
+    if c1 is not None:
+        c1 = {k2: v for k2, v in c1.items() if not k2.startswith("_")}
+    if spatial is not None:
+        spatial = {k2: v for k2, v in spatial.items() if not k2.startswith("_")}
 
     line = {
         "metric": "build ms @10M float3; kNN(k=8) queries/s; batched-vs-plain speedup; p99 ms",
@@ -526,12 +765,16 @@ def main():
             "batched_vs_plain_speedup": forest["batched_vs_plain_speedup"] if forest else None,
             "p99_ms": serve["serve_p99_ms"] if serve else None,
             "visits_per_query": round(visits / nq, 2),
+            "c1": c1,
+            "spatial": spatial,
             "forest": forest,
             "serve": serve,
         },
+        "parity": parity,
         "roofline": {"bound": "hbm", "achieved": round(knn_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(knn_gbs / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
+                     "traffic_source": traffic_note,
                      "kernel": "k_query2<3, FixedCandidateList<8>> (stack-based, two queries "
                                "per lane) on Morton-ordered queries",
                      "algorithmic_bytes_per_launch": knn_bytes,

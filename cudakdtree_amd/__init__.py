@@ -573,7 +573,9 @@ class SpatialKDTree:
 
     @property
     def bounds(self):
-        return list(self.raw.bounds[:2 * self.layout.num_dims])
+        """float bounds, or int32 ones for int points"""
+        b = np.ctypeslib.as_array(self.raw.bounds)[:2 * self.layout.num_dims].copy()
+        return list(b.view(self.layout.scalar_dtype))
 
     def nodes_numpy(self):
         """Copies the node array to the host as a structured array."""
@@ -581,7 +583,8 @@ class SpatialKDTree:
         n = self.raw.num_nodes
         buf = torch.empty(n * 12, dtype=torch.uint8, device=self.points.device)
         _copy_dev(buf, self.raw.nodes, n * 12)
-        dt = np.dtype([("pos", "<f4"), ("offset", "<u4"), ("count", "<u2"), ("dim", "<i2")])
+        pos = "<f4" if self.layout.scalar_type == SCALAR_F32 else "<i4"
+        dt = np.dtype([("pos", pos), ("offset", "<u4"), ("count", "<u2"), ("dim", "<i2")])
         return buf.cpu().numpy().view(dt)
 
     def prims_numpy(self):

@@ -313,12 +313,15 @@ __device__ inline int tiles_of(const LevelGeom &g, int s, int T)
 // its candidates; the last tile of each subtree picks the digit bin that
 // holds the pivot rank.
 //
-// Cross-workgroup protocol without device fences (an agent-scope fence
-// writes back and invalidates L2, which thrashes the cache for every other
-// tile): a tile adds its bins with value-returning atomics, so they have
-// been performed before the tile's barrier and before its thread 0 bumps
-// the subtree's tile counter; the tile that sees the final count reads and
-// clears the touched bins with atomic exchanges.
+// Cross-workgroup protocol: a tile adds its bins to the subtree's global
+// histogram with atomics; after the workgroup barrier its thread 0 bumps
+// the subtree's tile counter with an agent-scope release RMW.  The tile
+// that reads the final count issues an agent-scope acquire fence, so
+// (release sequence on the counter, fence synchronisation) every tile's
+// flush happens-before its read-back, which reads and clears the touched
+// bins with atomic exchanges.  Measured at 10M points: the release costs
+// 0.8 ms per build; the acquire, taken by one tile per subtree and pass,
+// costs about 0.1 ms.
 __global__ __launch_bounds__(SELECT_NT) void k_select_pass(
     TreeSet<LvlTree> set, int level, int word, int shift, int bits,
     const uint32_t *key, const uint8_t *data, const uint32_t *idx,
@@ -330,7 +333,6 @@ __global__ __launch_bounds__(SELECT_NT) void k_select_pass(
   __shared__ SegState s_st[2];
   __shared__ int s_seg[2];
   __shared__ int s_last[2];
-  __shared__ uint32_t s_sink;
   __shared__ uint32_t s_part[SELECT_NT];
   __shared__ uint32_t s_pick[3];
   __shared__ uint32_t s_bm[SEL_HCOPIES][DIGIT_MAX_BINS / 32];
@@ -358,7 +360,7 @@ __global__ __launch_bounds__(SELECT_NT) void k_select_pass(
   const int seg0 = g.seg_of(t0);
   const int seg1 = g.seg_of(t1 - 1);
   for (int i = threadIdx.x; i < 2 * nbins; i += blockDim.x) s_hist[i / nbins][i % nbins] = 0u;
-  if (threadIdx.x == 0) { s_seg[0] = seg0; s_seg[1] = seg1; s_st[0] = st[seg0]; s_sink = 0; }
+  if (threadIdx.x == 0) { s_seg[0] = seg0; s_seg[1] = seg1; s_st[0] = st[seg0]; }
   if (threadIdx.x == 1) s_st[1] = st[seg1];
   __syncthreads();
   // tiles inside one subtree use slot 0 only
@@ -416,30 +418,32 @@ __global__ __launch_bounds__(SELECT_NT) void k_select_pass(
   }
   __syncthreads();
   const int nslots = seg1 != seg0 ? 2 : 1;
-  uint32_t sink = 0;
   for (int sl = 0; sl < nslots; sl++) {
     const int s = s_seg[sl];
     uint32_t *h = hist + ((size_t)s * hcopies + hcopy) * HIST_STRIDE;
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
       const uint32_t c = s_hist[sl][b];
-      if (c) sink |= atomicAdd(&h[b], c);
+      if (c) atomicAdd(&h[b], c);
       // bins b..: lanes 0-31 and 32-63 of a wave cover two bitmap words
       const uint64_t m = __ballot(c != 0u);
       const uint32_t part = (lane < 32) ? (uint32_t)m : (uint32_t)(m >> 32);
-      if ((lane & 31) == 0 && part) sink |= atomicOr(&h[DIGIT_MAX_BINS + (b >> 5)], part);
+      if ((lane & 31) == 0 && part) atomicOr(&h[DIGIT_MAX_BINS + (b >> 5)], part);
     }
   }
-  // consuming the returned values makes every atomic complete before the
-  // barrier below
-  if (sink == 0xffffffffu) s_sink = sink;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int sl = 0; sl < 2; sl++) {
       s_last[sl] = 0;
       if (sl < nslots) {
         const int s = s_seg[sl];
-        const uint32_t done = atomicAdd(&st[s].tiles_done, 1u);
+        // release at agent scope: every thread's flush (ordered before
+        // this by the barrier) happens-before the RMW; the tile that reads
+        // the final count acquires, so all tiles' flushes happen-before
+        // its read-back (release sequence on tiles_done)
+        const uint32_t done = __hip_atomic_fetch_add(&st[s].tiles_done, 1u, __ATOMIC_RELEASE,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
         s_last[sl] = (int)done == tiles_of(g, s, T) - 1;
+        if (s_last[sl]) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
     }
   }

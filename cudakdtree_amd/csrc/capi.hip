@@ -668,10 +668,6 @@ int cukd_spatial_build(cukd_spatial_tree *tree, const void *d_points, int32_t n,
   g_last_error[0] = 0;
   if (int rc = check_layout(layout)) return rc;
   if (!tree) { set_last_error("tree is null"); return CUKD_ERR_INVALID; }
-  if (layout->scalar_type != CUKD_SCALAR_F32) {
-    set_last_error("spatial k-d trees need float coordinates");
-    return CUKD_ERR_UNSUPPORTED;
-  }
   if (n < 0 || (n > 0 && !d_points)) { set_last_error("bad points"); return CUKD_ERR_INVALID; }
   if (leaf_threshold > 65535) {
     set_last_error("leaf threshold %d exceeds the 16-bit leaf count", leaf_threshold);
@@ -703,7 +699,8 @@ static int spatial_query(int traversal, int list_kind, int k, const cukd_spatial
 {
   if (int rc = check_layout(layout)) return rc;
   if (layout->scalar_type != CUKD_SCALAR_F32) {
-    set_last_error("spatial k-d trees need float coordinates");
+    set_last_error("spatial queries need float coordinates (the reference defines no "
+                   "distance math for int points)");
     return CUKD_ERR_UNSUPPORTED;
   }
   if (traversal != CUKD_STACK_BASED && traversal != CUKD_CCT) {

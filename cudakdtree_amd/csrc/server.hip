@@ -5,8 +5,10 @@
 // of SERVER_SLOTS batch slots.  Each slot has a stream of its own and a
 // thread that waits for its batch and completes its requests, so batches
 // run side by side on the device: a batch held up by a few long traversals
-// does not hold up the ones admitted after it.  No counterpart in the
-// reference, whose
+// does not hold up the ones admitted after it.  A heavy request (many
+// queries times k) runs in a batch of its own, so the light requests it
+// would otherwise share a batch with do not wait for it; a request larger
+// than a slot is served in pieces.  No counterpart in the reference, whose
 // queries are per-thread device calls; every result equals the single-tree
 // cukd_fcp / cukd_knn result for the same queries.
 #include <atomic>
@@ -33,8 +35,20 @@ inline double ms_between(Clock::time_point a, Clock::time_point b)
   return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
+// what the caller submitted: completed once all of its pieces are
+struct Ticket {
+  uint64_t id;
+  int nq;
+  int pieces_left;
+  int status;
+  bool started;
+  Clock::time_point t_submit, t_start;
+};
+
+// one piece of a ticket's queries (the whole request unless it is larger
+// than a slot)
 struct Request {
-  uint64_t ticket;
+  Ticket *tk;
   int tree;
   const float *q;
   int nq;
@@ -42,18 +56,25 @@ struct Request {
   float cutoff;
   int32_t *ids;
   float *dist2;
-  Clock::time_point t_submit, t_start, t_done;
 };
-
-inline size_t request_bytes(int nq, int k, int D)
-{
-  const size_t kk = k > 0 ? (size_t)k : 1;
-  return (size_t)nq * (D * 4 + kk * 8) + 64;
-}
 
 inline size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
-enum { SERVER_SLOTS = 4 };
+// Bytes one (k, cutoff) class of a batch takes in the slot: nq queries in
+// nseg segments; the same regions, each 256-byte aligned, that
+// launch_batch lays out.
+inline size_t class_bytes(size_t nq, size_t nseg, int k, int D)
+{
+  const size_t kk = k > 0 ? (size_t)k : 1;
+  return align_up(nq * D * 4) + align_up((nseg + 1) * 4) + align_up(nseg * 4) +
+         2 * align_up(nq * kk * 4);
+}
+
+// Work of a request, in candidate-list slots: one with at least HEAVY_COST
+// runs in a batch of its own.
+inline size_t request_cost(int nq, int k) { return (size_t)nq * (size_t)(k > 0 ? k : 1); }
+
+enum { SERVER_SLOTS = 8, HEAVY_COST = 1 << 16 };
 
 // one batch: free -> packed by the worker -> launched -> completed by the
 // slot's own thread -> free (state guarded by the server's mutex)
@@ -208,18 +229,6 @@ struct cukd_server {
 
 namespace {
 
-// Records a batch error's message unless an earlier error is recorded
-// (called without the server's mutex; slots fail independently).
-void note_error(cukd_server *S, const char *fmt, ...)
-{
-  std::lock_guard<std::mutex> lk(S->mu);
-  if (S->error) return;
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(S->error_msg, sizeof(S->error_msg), fmt, ap);
-  va_end(ap);
-}
-
 // Packs the requests of one batch into a slot and launches it: one H2D
 // copy, one forest query launch per (k, cutoff) class, one D2H copy of the
 // results, then an event.
@@ -260,7 +269,8 @@ int launch_batch(cukd_server *S, Slot &sl)
     lay[c].d2 = o; o = align_up(o + (size_t)lay[c].nq * kk * 4);
   }
   if (o > sl.bytes) {
-    note_error(S, "batch of %zu bytes exceeds the slot", o);
+    // admission charges exactly this layout, so this is not reached
+    set_last_error("batch of %zu bytes exceeds the slot of %zu", o, sl.bytes);
     return CUKD_ERR_INVALID;
   }
   sl.res_begin = res_begin;
@@ -300,21 +310,22 @@ int launch_batch(cukd_server *S, Slot &sl)
       rc = cukd_forest_knn(S->traversal, CUKD_LIST_FIXED, cls[c].k, S->forest, S->points,
                            S->world_bounds, dq, lay[c].nq, lay[c].nseg, dso, dst, cls[c].cutoff,
                            ids, d2, nullptr, 0, sl.stream);
-    if (rc != CUKD_OK) {
-      note_error(S, "batch launch: %s", cukd_last_error());
-      return rc;
-    }
+    if (rc != CUKD_OK) return rc;
   }
   // the results' copy is queued by the slot's thread once the queries have
   // run: copies queued behind a long batch's kernels would hold up the
   // copies of the batches launched after it
   if (e == hipSuccess) e = hipEventRecord(sl.kdone, sl.stream);
   if (e != hipSuccess) {
-    note_error(S, "batch copy: %s", hipGetErrorString(e));
+    set_last_error("batch copy: %s", hipGetErrorString(e));
     return CUKD_ERR_HIP;
   }
   return CUKD_OK;
 }
+
+// Completes one piece (the server's mutex held): its ticket completes with
+// its last piece.
+void complete_piece(cukd_server *S, Request *r, int status, Clock::time_point now);
 
 // Waits for a slot's batch, copies its results back and on to every
 // request's caller, and records the completions.
@@ -339,21 +350,9 @@ int finish_batch(cukd_server *S, Slot &sl)
     S->copies->run(jobs);
   }
   const Clock::time_point now = Clock::now();
-  for (size_t i = 0; i < nreq; i++) sl.reqs[i]->t_done = now;
   std::lock_guard<std::mutex> lk(S->mu);
-  for (size_t i = 0; i < nreq; i++) {
-    Request *r = sl.reqs[i];
-    cukd_completion c;
-    c.ticket = r->ticket;
-    c.status = e == hipSuccess ? CUKD_OK : CUKD_ERR_HIP;
-    c.num_queries = r->nq;
-    c.queue_ms = (float)ms_between(r->t_submit, r->t_start);
-    c.total_ms = (float)ms_between(r->t_submit, r->t_done);
-    S->done.push_back(c);
-    S->stats.requests_done++;
-    S->stats.queries_done += (uint64_t)r->nq;
-    delete r;
-  }
+  for (size_t i = 0; i < nreq; i++)
+    complete_piece(S, sl.reqs[i], e == hipSuccess ? CUKD_OK : CUKD_ERR_HIP, now);
   sl.reqs.clear();
   sl.state = SLOT_FREE;
   S->cv_work.notify_one();
@@ -363,6 +362,24 @@ int finish_batch(cukd_server *S, Slot &sl)
   }
   S->cv_done.notify_all();
   return e == hipSuccess ? CUKD_OK : CUKD_ERR_HIP;
+}
+
+void complete_piece(cukd_server *S, Request *r, int status, Clock::time_point now)
+{
+  Ticket *tk = r->tk;
+  if (status != CUKD_OK && tk->status == CUKD_OK) tk->status = status;
+  S->stats.queries_done += (uint64_t)r->nq;
+  delete r;
+  if (--tk->pieces_left > 0) return;
+  cukd_completion c;
+  c.ticket = tk->id;
+  c.status = tk->status;
+  c.num_queries = tk->nq;
+  c.queue_ms = (float)ms_between(tk->t_submit, tk->t_start);
+  c.total_ms = (float)ms_between(tk->t_submit, now);
+  S->done.push_back(c);
+  S->stats.requests_done++;
+  delete tk;
 }
 
 // The worker: admits queued requests into a free slot and launches the
@@ -385,17 +402,37 @@ void worker_main(cukd_server *S)
       if (S->pending.empty()) break;  // stopping, and every request admitted
       si = free_slot();
       Slot &sl = S->slot[si];
-      // admit requests in arrival order while they fit the slot
+      // admit requests in arrival order while the batch, laid out exactly
+      // as launch_batch lays it out, fits the slot; a heavy request only
+      // as the first of a batch, which it then closes
+      struct Cls { int k; float cutoff; size_t nq, nseg, bytes; };
+      std::vector<Cls> cls;
       size_t bytes = 0;
       const Clock::time_point now = Clock::now();
       while (!S->pending.empty() && (S->max_per_batch <= 0 || (int)sl.reqs.size() < S->max_per_batch)) {
         Request *r = S->pending.front();
-        const size_t b = request_bytes(r->nq, r->k, S->D) + 4 * 256;
-        if (!sl.reqs.empty() && bytes + b > S->slot_bytes) break;
-        bytes += b;
-        r->t_start = now;
+        const bool heavy = request_cost(r->nq, r->k) >= (size_t)HEAVY_COST;
+        if (heavy && !sl.reqs.empty()) break;
+        size_t c = 0;
+        for (; c < cls.size(); c++)
+          if (cls[c].k == r->k && memcmp(&cls[c].cutoff, &r->cutoff, 4) == 0) break;
+        const size_t old_b = c < cls.size() ? cls[c].bytes : 0;
+        const size_t nq = (c < cls.size() ? cls[c].nq : 0) + (size_t)r->nq;
+        const size_t nseg = (c < cls.size() ? cls[c].nseg : 0) + 1;
+        const size_t new_b = class_bytes(nq, nseg, r->k, S->D);
+        if (!sl.reqs.empty() && bytes - old_b + new_b > S->slot_bytes) break;
+        if (c == cls.size()) cls.push_back(Cls{r->k, r->cutoff, 0, 0, 0});
+        cls[c].nq = nq;
+        cls[c].nseg = nseg;
+        cls[c].bytes = new_b;
+        bytes = bytes - old_b + new_b;
+        if (!r->tk->started) {
+          r->tk->started = true;
+          r->tk->t_start = now;
+        }
         sl.reqs.push_back(r);
         S->pending.pop_front();
+        if (heavy) break;
       }
       sl.state = SLOT_PACKING;
     }
@@ -403,19 +440,16 @@ void worker_main(cukd_server *S)
     const int rc = launch_batch(S, sl);
     std::lock_guard<std::mutex> lk(S->mu);
     if (rc != CUKD_OK) {
-      // fail every request of the batch
-      for (Request *r : sl.reqs) {
-        cukd_completion c;
-        c.ticket = r->ticket;
-        c.status = CUKD_ERR_HIP;
-        c.num_queries = r->nq;
-        c.queue_ms = c.total_ms = 0.f;
-        S->done.push_back(c);
-        delete r;
-      }
+      // fail the batch's requests; only a failed HIP call marks the server
+      // itself as failed
+      const Clock::time_point now = Clock::now();
+      for (Request *r : sl.reqs) complete_piece(S, r, rc, now);
       sl.reqs.clear();
       sl.state = SLOT_FREE;
-      if (!S->error) S->error = CUKD_ERR_HIP;
+      if (rc == CUKD_ERR_HIP && !S->error) {
+        S->error = CUKD_ERR_HIP;
+        snprintf(S->error_msg, sizeof(S->error_msg), "batch launch: %s", cukd_last_error());
+      }
       S->cv_done.notify_all();
       continue;
     }
@@ -527,29 +561,46 @@ int cukd_server_submit(cukd_server *S, int32_t tree, const float *h_queries, int
   if (nq < 1 || !h_queries || !h_ids) { set_last_error("empty request"); return CUKD_ERR_INVALID; }
   if (k < 0 || k > 64) { set_last_error("k = %d not in 0..64", k); return CUKD_ERR_INVALID; }
   if (k > 0 && !h_dist2) { set_last_error("kNN needs h_dist2"); return CUKD_ERR_INVALID; }
-  if (request_bytes(nq, k, S->D) + 4 * 256 > S->slot_bytes) {
-    set_last_error("request of %d queries exceeds the server's memory budget", nq);
+  // pieces of at most 'piece' queries, each of which fits a slot alone
+  int piece = nq;
+  while (piece > 1 && class_bytes((size_t)piece, 1, k, S->D) > S->slot_bytes) piece = (piece + 1) / 2;
+  if (class_bytes((size_t)piece, 1, k, S->D) > S->slot_bytes) {
+    set_last_error("the server's memory budget cannot hold even one query");
     return CUKD_ERR_INVALID;
   }
-  Request *r = new Request();
-  r->tree = tree;
-  r->q = h_queries;
-  r->nq = nq;
-  r->k = k;
-  r->cutoff = cutoff;
-  r->ids = h_ids;
-  r->dist2 = h_dist2;
-  r->t_submit = Clock::now();
+  const int pieces = (nq + piece - 1) / piece;
+  Ticket *tk = new Ticket();
+  tk->nq = nq;
+  tk->pieces_left = pieces;
+  tk->status = CUKD_OK;
+  tk->started = false;
+  tk->t_submit = Clock::now();
+  const size_t kk = k > 0 ? (size_t)k : 1;
+  std::vector<Request *> rs((size_t)pieces);
+  for (int p = 0; p < pieces; p++) {
+    const int a = p * piece, n = nq - a < piece ? nq - a : piece;
+    Request *r = new Request();
+    r->tk = tk;
+    r->tree = tree;
+    r->q = h_queries + (size_t)a * S->D;
+    r->nq = n;
+    r->k = k;
+    r->cutoff = cutoff;
+    r->ids = h_ids + (size_t)a * kk;
+    r->dist2 = h_dist2 ? h_dist2 + (size_t)a * kk : nullptr;
+    rs[p] = r;
+  }
   {
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->stop) {
-      delete r;
+      for (Request *r : rs) delete r;
+      delete tk;
       set_last_error("server is shutting down");
       return CUKD_ERR_INVALID;
     }
-    r->ticket = S->next_ticket++;
-    *ticket = r->ticket;
-    S->pending.push_back(r);
+    tk->id = S->next_ticket++;
+    *ticket = tk->id;
+    for (Request *r : rs) S->pending.push_back(r);
     S->stats.requests_submitted++;
   }
   S->cv_work.notify_one();

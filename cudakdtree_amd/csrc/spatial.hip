@@ -42,6 +42,24 @@ __device__ inline float abox_dec(int32_t b)
   return __int_as_float(b);
 }
 
+// A coordinate as the builder sees it: int coordinates go through the
+// reference's implicit int -> float conversion (AtomicBox::encode(float),
+// 'const float center = get_coord(point,split.dim)',
+// spatial-kdtree.h:172-188, 427-429).
+__device__ inline float sp_coord(const uint8_t *pts, uint32_t i, const cukd_layout &L, int d)
+{
+  const uint32_t b = load_coord_bits(pts + (size_t)i * L.stride, L, d);
+  return L.scalar_type == CUKD_SCALAR_F32 ? __uint_as_float(b) : (float)(int32_t)b;
+}
+
+// Node::pos and the bounds hold scalar_t: a float result stored into an
+// int one is truncated by C's conversion for int points
+// (spatial-kdtree.h:75-77, 163-168, 489).
+__device__ inline uint32_t sp_store(float f, bool isf)
+{
+  return isf ? __float_as_uint(f) : (uint32_t)(int32_t)f;
+}
+
 enum { SP_OPEN = 0, SP_LEAF = 1, SP_SPLIT = 2, SP_INNER = 3 };
 
 struct SpNode {
@@ -158,9 +176,8 @@ __global__ __launch_bounds__(256) void k_sp_root_bounds(SpNode *nodes, const uin
   int32_t lo[CUKD_MAX_DIMS], hi[CUKD_MAX_DIMS];
   for (int d = 0; d < D; d++) { lo[d] = 0x7fffffff; hi[d] = (int32_t)0x80000000; }
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const float *p = (const float *)(pts + (size_t)i * L.stride + L.point_offset);
     for (int d = 0; d < D; d++) {
-      const int32_t e = abox_enc(p[d]);
+      const int32_t e = abox_enc(sp_coord(pts, (uint32_t)i, L, d));
       lo[d] = min(lo[d], e);
       hi[d] = max(hi[d], e);
     }
@@ -253,8 +270,7 @@ __global__ void k_sp_classify(const SpNode *nodes, const uint32_t *pos_prim,
     if (t.dim < 0) {
       side = (int)(((uint32_t)p - t.start) & 1u);
     } else {
-      const float *pt = (const float *)(pts + (size_t)pos_prim[p] * L.stride + L.point_offset);
-      side = pt[t.dim] >= t.pos;
+      side = sp_coord(pts, pos_prim[p], L, t.dim) >= t.pos;
     }
     left = side == 0;
   }
@@ -291,10 +307,9 @@ __global__ void k_sp_scatter(SpNode *nodes, const uint32_t *pos_prim, const uint
   }
   npos_prim[np] = prim;
   npos_node[np] = child;
-  const float *pt = (const float *)(pts + (size_t)prim * L.stride + L.point_offset);
   SpNode &c = nodes[child];
   for (int d = 0; d < L.num_dims; d++) {
-    const int32_t e = abox_enc(pt[d]);
+    const int32_t e = abox_enc(sp_coord(pts, prim, L, d));
     atomicMin(&c.lo[d], e);
     atomicMax(&c.hi[d], e);
   }
@@ -339,7 +354,7 @@ __global__ void k_sp_write_prims(const SpNode *nodes, const uint32_t *leaf_off,
 
 // writeNodes (spatial-kdtree.h:477-494): dim -1 is stored as 0
 __global__ void k_sp_write_nodes(const SpNode *nodes, const uint32_t *leaf_off,
-                                 cukd_spatial_node *out, int num_nodes)
+                                 cukd_spatial_node *out, int num_nodes, bool isf)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= num_nodes) return;
@@ -354,17 +369,18 @@ __global__ void k_sp_write_nodes(const SpNode *nodes, const uint32_t *leaf_off,
     o.offset = t.offset;
     o.count = 0;
     o.dim = (int16_t)(t.dim < 0 ? 0 : t.dim);
-    o.pos = t.pos;
+    o.ipos = (int32_t)sp_store(t.pos, isf);
   }
   out[i] = o;
 }
 
-__global__ void k_sp_bounds_out(const SpNode *nodes, float *bounds, int D)
+// saveBounds (spatial-kdtree.h:464-473): box_t of scalar_t
+__global__ void k_sp_bounds_out(const SpNode *nodes, uint32_t *bounds, int D, bool isf)
 {
   const int d = threadIdx.x;
   if (d < D) {
-    bounds[d] = abox_dec(nodes[0].lo[d]);
-    bounds[D + d] = abox_dec(nodes[0].hi[d]);
+    bounds[d] = sp_store(abox_dec(nodes[0].lo[d]), isf);
+    bounds[D + d] = sp_store(abox_dec(nodes[0].hi[d]), isf);
   }
 }
 
@@ -392,7 +408,8 @@ int spatial_build_impl(cukd_spatial_tree *tree, const void *d_points, int n, con
   uint32_t *pflag = (uint32_t *)(tmp + o_pflag), *pexcl = (uint32_t *)(tmp + o_pexcl);
   uint32_t *scratch = (uint32_t *)(tmp + o_scan);
   uint32_t *num_split = (uint32_t *)(tmp + o_misc);
-  float *bounds_dev = (float *)(tmp + o_misc + 64);
+  uint32_t *bounds_dev = (uint32_t *)(tmp + o_misc + 64);
+  const bool isf = L.scalar_type == CUKD_SCALAR_F32;
   const uint8_t *pts = (const uint8_t *)d_points;
   int rc = CUKD_OK;
   hipError_t e = hipSuccess;
@@ -451,14 +468,14 @@ int spatial_build_impl(cukd_spatial_tree *tree, const void *d_points, int n, con
                            (const uint32_t *)nexcl, (const uint32_t *)pp[cur],
                            (const uint32_t *)pn[cur], out_prims, n);
       hipLaunchKernelGGL(k_sp_write_nodes, dim3(g), dim3(256), 0, s, (const SpNode *)nodes,
-                         (const uint32_t *)nexcl, out_nodes, num_nodes);
+                         (const uint32_t *)nexcl, out_nodes, num_nodes, isf);
       hipLaunchKernelGGL(k_sp_bounds_out, dim3(1), dim3(64), 0, s, (const SpNode *)nodes,
-                         bounds_dev, D);
-      float hb[2 * CUKD_MAX_DIMS];
-      e = hipMemcpyAsync(hb, bounds_dev, sizeof(float) * 2 * D, hipMemcpyDeviceToHost, s);
+                         bounds_dev, D, isf);
+      uint32_t hb[2 * CUKD_MAX_DIMS];
+      e = hipMemcpyAsync(hb, bounds_dev, 4 * 2 * D, hipMemcpyDeviceToHost, s);
       if (e == hipSuccess) e = hipStreamSynchronize(s);
       if (e == hipSuccess) {
-        for (int i = 0; i < 2 * CUKD_MAX_DIMS; i++) tree->bounds[i] = i < 2 * D ? hb[i] : 0.f;
+        for (int i = 0; i < 2 * CUKD_MAX_DIMS; i++) tree->ibounds[i] = i < 2 * D ? (int32_t)hb[i] : 0;
         tree->nodes = out_nodes;
         tree->prim_ids = out_prims;
         tree->data = d_points;

@@ -195,8 +195,7 @@ inline void buildTree(SpatialKDTree<data_t, data_traits> &tree, data_t *d_points
                       BuildConfig buildConfig = {}, hipStream_t stream = 0,
                       GpuMemoryResource &memResource = defaultGpuMemResource())
 {
-  static_assert(std::is_same<typename SpatialKDTree<data_t, data_traits>::scalar_t, float>::value,
-                "cukd: spatial k-d trees need float coordinates");
+  using scalar_t = typename SpatialKDTree<data_t, data_traits>::scalar_t;
   const cukd_layout L = layout_of<data_t, data_traits>();
   cukd_mem_resource mr = detail::bridge(memResource);
   cukd_spatial_tree raw;
@@ -204,9 +203,14 @@ inline void buildTree(SpatialKDTree<data_t, data_traits> &tree, data_t *d_points
                                    stream, &mr),
                 "buildTree(SpatialKDTree)");
   enum { D = point_traits<typename data_traits::point_t>::num_dims };
+  // float points: float bounds; int points: the int32 bounds
   for (int d = 0; d < D; d++) {
-    point_traits<typename data_traits::point_t>::set_coord(tree.bounds.lower, d, raw.bounds[d]);
-    point_traits<typename data_traits::point_t>::set_coord(tree.bounds.upper, d, raw.bounds[D + d]);
+    const scalar_t lo = std::is_same<scalar_t, float>::value ? (scalar_t)raw.bounds[d]
+                                                             : (scalar_t)raw.ibounds[d];
+    const scalar_t hi = std::is_same<scalar_t, float>::value ? (scalar_t)raw.bounds[D + d]
+                                                             : (scalar_t)raw.ibounds[D + d];
+    point_traits<typename data_traits::point_t>::set_coord(tree.bounds.lower, d, lo);
+    point_traits<typename data_traits::point_t>::set_coord(tree.bounds.upper, d, hi);
   }
   tree.nodes = (typename SpatialKDTree<data_t, data_traits>::Node *)raw.nodes;
   tree.primIDs = raw.prim_ids;

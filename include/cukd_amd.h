@@ -224,17 +224,24 @@ int cukd_knn_extract(const uint64_t *d_lists, int32_t nq, int k,
                      int32_t *d_ids, float *d_dist2, float *d_radius,
                      hipStream_t stream);
 
-/* Spatial k-d tree (spatial-kdtree.h:69-98).  Node = {float pos;
- * uint32 offset; uint16 count; int16 dim} (12 bytes). */
+/* Spatial k-d tree (spatial-kdtree.h:69-98).  Node = {scalar_t pos;
+ * uint32 offset; uint16 count; int16 dim} (12 bytes): pos and the bounds
+ * are float for float points and int32 for int points (ipos, ibounds). */
 typedef struct cukd_spatial_node {
-  float pos;
+  union {
+    float pos;
+    int32_t ipos;
+  };
   uint32_t offset;
   uint16_t count;
   int16_t dim;
 } cukd_spatial_node;
 
 typedef struct cukd_spatial_tree {
-  float bounds[2 * 8];           /* lower[num_dims] then upper[num_dims] */
+  union {                        /* lower[num_dims] then upper[num_dims] */
+    float bounds[2 * 8];
+    int32_t ibounds[2 * 8];
+  };
   cukd_spatial_node *nodes;      /* device, num_nodes entries            */
   uint32_t *prim_ids;            /* device, num_prims entries            */
   const void *data;              /* device, the (unpermuted) points      */
@@ -244,6 +251,10 @@ typedef struct cukd_spatial_tree {
 
 /* Builds a spatial k-d tree (spatial-kdtree.h:113-120, 496-612, 632-645).
  * leaf_threshold 0 means 8 (BuildConfig::makeLeafThreshold, :100-105).
+ * Int points are built as the reference builds them: the split decisions
+ * see each coordinate converted to float, and pos and the bounds are that
+ * float value converted back to int32 (truncated).  Queries need float
+ * points (the reference has no distance math for int points).
  * Synchronises 'stream' before returning, as the reference does, because
  * the node count and bounds are returned in host memory. */
 int cukd_spatial_build(cukd_spatial_tree *tree, const void *d_points,

@@ -203,7 +203,9 @@ def spatial_build(points, layout, leaf_threshold=0):
                                  None, None, None)
     nodes = np.zeros(num, SPATIAL_NODE_DTYPE)
     prims = np.zeros(max(n, 1), np.uint32)
-    bounds = np.zeros(2 * layout.num_dims, np.float32)
+    # box_t<point_t>: float, or int32 for int points (pos likewise holds
+    # int32 bits then)
+    bounds = np.zeros(2 * layout.num_dims, _scalar_dtype(layout))
     lib().oc_spatial_build(_ptr(pts), n, ctypes.byref(lay), leaf_threshold,
                            _ptr(nodes), _ptr(prims), _ptr(bounds))
     return nodes, prims[:n], bounds

@@ -725,6 +725,30 @@ static inline float abox_dec(int32_t bits)
   return f;
 }
 
+/* A coordinate as the spatial builder sees it: int coordinates go through
+ * the implicit int -> float conversion of AtomicBox::encode(float) and of
+ * 'const float center = get_coord(point,split.dim)'
+ * (spatial-kdtree.h:172-188, 199-207, 427-429). */
+static inline float sp_coord(const uint8_t *e, const oc_layout *L, int d)
+{
+  return L->scalar_type == OC_F32 ? get_f(e, L, d) : (float)get_i(e, L, d);
+}
+
+/* Node::pos and the tree's box_t hold scalar_t: a float result is stored
+ * into an int one by C's float -> int conversion (truncation) for int
+ * points (spatial-kdtree.h:75-77, 163-168, 489). */
+static inline uint32_t sp_store(float f, const oc_layout *L)
+{
+  uint32_t bits;
+  if (L->scalar_type == OC_F32) {
+    memcpy(&bits, &f, 4);
+  } else {
+    const int32_t i = (int32_t)f;
+    memcpy(&bits, &i, 4);
+  }
+  return bits;
+}
+
 typedef struct {
   int32_t lo[OC_MAX_DIMS], hi[OC_MAX_DIMS];
   uint32_t count;
@@ -758,17 +782,19 @@ int64_t oc_spatial_build(const void *points, int64_t n, const oc_layout *L,
     prim_node[i] = 0;
     tn[0].count++;
     for (int d = 0; d < D; d++) {
-      const int32_t v = abox_enc(get_f(base + i * L->stride, L, d));
+      const int32_t v = abox_enc(sp_coord(base + i * L->stride, L, d));
       if (v < tn[0].lo[d]) tn[0].lo[d] = v;
       if (v > tn[0].hi[d]) tn[0].hi[d] = v;
     }
   }
-  /* saveBounds (spatial-kdtree.h:464-473) */
-  if (bounds_out)
+  /* saveBounds (spatial-kdtree.h:464-473): box_t of scalar_t */
+  if (bounds_out) {
+    uint32_t *bo = (uint32_t *)bounds_out;
     for (int d = 0; d < D; d++) {
-      bounds_out[d] = abox_dec(tn[0].lo[d]);
-      bounds_out[D + d] = abox_dec(tn[0].hi[d]);
+      bo[d] = sp_store(abox_dec(tn[0].lo[d]), L);
+      bo[D + d] = sp_store(abox_dec(tn[0].hi[d]), L);
     }
+  }
 
   int64_t first_open = 0, end_open = 1;
   while (first_open < end_open) {
@@ -812,21 +838,15 @@ int64_t oc_spatial_build(const void *points, int64_t n, const oc_layout *L,
     for (int64_t i = 0; i < n; i++) {
       sp_tmp *t = &tn[prim_node[i]];
       if (t->state != 2) continue;
-      const float *pp = (const float *)(base + i * L->stride + L->point_offset);
+      const uint8_t *pe = base + i * L->stride;
       int side;
       if (t->dim == -1) side = (int)(split_rank[prim_node[i]]++ & 1u);
-      else {
-        float c;
-        memcpy(&c, pp + t->dim, 4);
-        side = c >= t->pos;
-      }
+      else side = sp_coord(pe, L, t->dim) >= t->pos;
       const uint32_t child = t->offset + (uint32_t)side;
       sp_tmp *ct = &tn[child];
       ct->count++;
       for (int d = 0; d < D; d++) {
-        float f;
-        memcpy(&f, pp + d, 4);
-        const int32_t v = abox_enc(f);
+        const int32_t v = abox_enc(sp_coord(pe, L, d));
         if (v < ct->lo[d]) ct->lo[d] = v;
         if (v > ct->hi[d]) ct->hi[d] = v;
       }
@@ -855,7 +875,8 @@ int64_t oc_spatial_build(const void *points, int64_t n, const oc_layout *L,
       o->offset = is_leaf_with_prims ? (uint32_t)start[j] : t->offset;
       o->count = (uint16_t)t->dcount;
       o->dim = (int16_t)(t->dim == -1 ? 0 : t->dim); /* writeNodes :491-493 */
-      o->pos = t->pos;
+      const uint32_t pb = sp_store(t->pos, L);
+      memcpy(&o->pos, &pb, 4);
     }
     free(start);
     free(fill);

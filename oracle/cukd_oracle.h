@@ -75,6 +75,7 @@ void oc_knn_extract(const uint64_t *lists, int64_t nq, int k, int32_t *ids, floa
                     float *radius);
 
 /* ---- spatial k-d tree (spatial-kdtree.h:69-105, 496-612) ---- */
+/* pos (and bounds_out) hold scalar_t bits: float, or int32 for int points */
 typedef struct oc_spatial_node {
   float    pos;
   uint32_t offset;

@@ -62,6 +62,45 @@ def test_forest_build_matches_oracle_per_tree(kind):
     f.close()
 
 
+def _finish_blocks_at_largest_cap(sizes, explicit):
+    """Finish workgroups of a forest if every tree took the largest LDS
+    capacity (the forest plan's rule: 4096 for explicit dims, 8192
+    otherwise); 512 or more puts every tree on that capacity."""
+    cap = 4096 if explicit else 8192
+    total = 0
+    for n in sizes:
+        if n < 1:
+            continue
+        h = int(np.floor(np.log2(n - 1))) + 1 if n > 1 else 1
+        total += 1 << max(0, h - int(np.log2(cap)))
+    return total
+
+
+@pytest.mark.parametrize("kind", ["explicit", "photon"])
+def test_explicit_forest_with_many_finish_blocks(kind):
+    """An explicit-dim forest large enough (>= 512 finish workgroups) that
+    every tree finishes from 4096-element subtrees: each tree equals the
+    oracle's buildTree_host."""
+    D = 3
+    sizes = log_uniform_sizes(40, 20000, 200000, 61)
+    assert _finish_blocks_at_largest_cap(sizes, True) >= 512
+    off = offsets_of(sizes)
+    coords = forest_points(sizes, D, 63, dist="gmm")
+    lay = layout_for(kind, D)
+    elems = make_elements(coords, lay)
+    f = ck.Forest(off, lay)
+    pts = torch.from_numpy(elems).cuda()
+    wb = f.bounds_tensor()
+    f.build(pts, wb)
+    torch.cuda.synchronize()
+    got = pts.cpu().numpy()
+    for t in range(len(sizes)):
+        a, b = off[t], off[t + 1]
+        ref, _ = oc.build_host(elems[a:b], lay, want_bounds=True)
+        assert np.array_equal(got[a:b], ref), "tree %d (n=%d)" % (t, b - a)
+    f.close()
+
+
 def test_forest_equals_per_tree_builds_c4_shape():
     """512 trees, sizes log-uniform in [64, 262144]: the batched build and
     one cukd_build_tree per tree give identical arrays."""
@@ -137,11 +176,26 @@ def test_forest_queries_equal_per_tree_queries(method):
     q = torch.from_numpy(np.concatenate(qs)).cuda()
     d_off = torch.from_numpy(seg_off).cuda()
     d_tree = torch.from_numpy(seg_tree).cuda()
+    pts_h = pts.cpu().numpy()
+    wb_h = wb.cpu().numpy()
+    q_h = q.cpu().numpy()
     for k in (0, 1, 8, 20):
         if k == 0:
             ids, d2 = f.fcp(pts, q, d_off, d_tree, traversal=method, cutoff=0.5, world_bounds=wb)
         else:
             ids, d2, r2 = f.knn(pts, q, k, d_off, d_tree, traversal=method, world_bounds=wb)
+        # the oracle on each segment's own tree (ids local to that tree)
+        ids_h, d2_h = ids.cpu().numpy(), d2.cpu().numpy()
+        for s, t in enumerate(seg_tree):
+            a, b = int(seg_off[s]), int(seg_off[s + 1])
+            tree_h = pts_h[off[t]:off[t + 1]]
+            if k == 0:
+                oid, od2 = oc.fcp(method, tree_h, lay, q_h[a:b], cutoff=0.5, world_bounds=wb_h[t])
+            else:
+                oid, od2, _ = oc.knn(method, oc.LIST_FIXED, k, tree_h, lay, q_h[a:b],
+                                     world_bounds=wb_h[t])
+            assert np.array_equal(ids_h[a:b], oid), ("oracle ids", k, s)
+            assert np.array_equal(d2_h[a:b].view(np.uint32), od2.view(np.uint32)), ("oracle", k, s)
         for s, t in enumerate(seg_tree):
             a, b = int(seg_off[s]), int(seg_off[s + 1])
             tree = pts[off[t]:off[t + 1]]

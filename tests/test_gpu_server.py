@@ -8,6 +8,7 @@ import pytest
 import torch
 
 import cudakdtree_amd as ck
+import oracle as oc
 from cudakdtree_amd import datagen
 from tests.test_gpu_forest import forest_points, log_uniform_sizes, offsets_of
 
@@ -130,11 +131,86 @@ def test_concurrent_submitters(served_forest):
     srv.close()
 
 
+def _oracle(pts_h, off, lay, tree, q, k, cutoff):
+    t = pts_h[off[tree]:off[tree + 1]]
+    if k == 0:
+        return oc.fcp(oc.STACK_BASED, t, lay, q, cutoff=cutoff)
+    ids, d2, _ = oc.knn(oc.STACK_BASED, oc.LIST_FIXED, k, t, lay, q, cutoff=cutoff)
+    return ids, d2
+
+
+def test_burst_matches_oracle(served_forest):
+    """Every served request equals the oracle's stack-based fcp / knn
+    (FixedCandidateList<k>) on that request's own tree: ids and dist2
+    bits."""
+    f, pts, off, lay = served_forest
+    pts_h = pts.cpu().numpy()
+    reqs = _requests(off, 40, 17)
+    srv = ck.Server(f, pts, budget_bytes=64 << 20)
+    got = _serve(srv, reqs)
+    srv.close()
+    for (tree, q, k, cutoff), (ids, d2) in zip(reqs, got):
+        eids, ed2 = _oracle(pts_h, off, lay, tree, q, k, cutoff)
+        assert np.array_equal(ids.ravel(), eids.ravel())
+        assert np.array_equal(d2.view(np.uint32).ravel(), ed2.view(np.uint32).ravel())
+
+
+def test_small_budget_many_classes(served_forest):
+    """A 1 MiB budget (128 KiB slots) and hundreds of one-query requests,
+    each with a cutoff of its own (one class each): admission charges the
+    exact padded layout, so every batch fits and every request succeeds."""
+    f, pts, off, lay = served_forest
+    pts_h = pts.cpu().numpy()
+    srv = ck.Server(f, pts, budget_bytes=1 << 20)
+    T = len(off) - 1
+    reqs = []
+    for i in range(400):
+        k = (0, 1, 4, 16)[i % 4]
+        reqs.append((i % T, datagen.gmm(1, 3, 900 + i), k, 0.5 + i * 1e-3))
+    got = _serve(srv, reqs)
+    st = srv.stats()
+    srv.close()
+    assert st.requests_done == len(reqs) and st.batches < len(reqs)
+    for (tree, q, k, cutoff), (ids, d2) in zip(reqs, got):
+        eids, ed2 = _oracle(pts_h, off, lay, tree, q, k, cutoff)
+        assert np.array_equal(ids.ravel(), eids.ravel())
+        assert np.array_equal(d2.view(np.uint32).ravel(), ed2.view(np.uint32).ravel())
+
+
+def test_request_larger_than_a_slot_is_served_in_pieces(served_forest):
+    """A request that does not fit one slot is queued in pieces, not
+    refused, and its results equal the single-tree query."""
+    f, pts, off, lay = served_forest
+    srv = ck.Server(f, pts, budget_bytes=1 << 20)
+    big = datagen.gmm(30000, 3, 5)          # 30000 * (12 + 128) bytes >> 128 KiB
+    light = [(1, datagen.gmm(50, 3, 6), 4, np.inf), (2, datagen.gmm(7, 3, 8), 0, np.inf)]
+    reqs = [(0, big, 16, np.inf)] + light
+    got = _serve(srv, reqs)
+    st = srv.stats()
+    srv.close()
+    assert st.requests_done == 3 and st.batches >= 3
+    for (tree, q, k, cutoff), (ids, d2) in zip(reqs, got):
+        eids, ed2 = _expected(pts, off, lay, tree, q, k, cutoff)
+        assert np.array_equal(ids.ravel(), eids.ravel())
+        assert np.array_equal(d2.view(np.uint32).ravel(), ed2.view(np.uint32).ravel())
+
+
+def test_heavy_request_batched_alone(served_forest):
+    """A heavy request (nq * k >= 65536) is not batched with the light
+    requests queued with it."""
+    f, pts, off, lay = served_forest
+    srv = ck.Server(f, pts, budget_bytes=256 << 20)
+    reqs = [(0, datagen.gmm(10, 3, 1), 4, np.inf), (3, datagen.gmm(8000, 3, 2), 16, np.inf),
+            (1, datagen.gmm(10, 3, 3), 4, np.inf), (2, datagen.gmm(20, 3, 4), 0, np.inf)]
+    _serve(srv, reqs)
+    st = srv.stats()
+    srv.close()
+    assert st.requests_done == 4 and st.batches >= 2
+
+
 def test_budget_and_argument_errors(served_forest):
     f, pts, off, lay = served_forest
     srv = ck.Server(f, pts, budget_bytes=1 << 20)
-    with pytest.raises(ck.CukdError):
-        srv.submit(0, datagen.uniform(100000, 3, 1), 16)  # larger than a quarter of the budget
     with pytest.raises(ck.CukdError):
         srv.submit(len(off) - 1, datagen.uniform(10, 3, 1), 4)  # no such tree
     with pytest.raises(ck.CukdError):

@@ -90,3 +90,34 @@ def test_spatial_knn16_cutoff_config_shape():
     assert np.array_equal(gid.cpu().numpy(), rid)
     assert np.array_equal(gd2.cpu().numpy().view(np.uint32), rd2.view(np.uint32))
     ck.free(t)
+
+
+def _int3_coords(n, seed, dup_every=0):
+    """int3 coordinates uniform in [0, 2^20) (the spatial config's int
+    set), optionally with exact duplicates."""
+    c = (datagen.splitmix64(seed, n * 3) >> np.uint64(44)).astype(np.int32).reshape(n, 3)
+    if dup_every:
+        m = n // dup_every
+        src = (datagen.splitmix64(seed + 1, m) % np.uint64(n)).astype(np.int64)
+        c[n - m:] = c[src]
+    return c
+
+
+@pytest.mark.parametrize("n", [1, 9, 1000, 100_000, 2_000_000])
+def test_spatial_build_int3(n):
+    """Spatial trees over int points (Node::pos is scalar_t,
+    spatial-kdtree.h:69-98): nodes (int32 pos), primIDs and bounds equal
+    the oracle's; 2M int3 with 1% duplicates is the config's int set."""
+    coords = _int3_coords(n, 31 + n, dup_every=100 if n >= 1000 else 0)
+    lay = ck.int_points(3)
+    rn, rp, rb = oc.spatial_build(coords, lay)
+    t = ck.buildSpatialTree(torch.from_numpy(coords).cuda(), lay)
+    assert t.num_nodes == len(rn)
+    assert np.array_equal(t.nodes_numpy().view(np.uint8), rn.view(np.uint8))
+    assert np.array_equal(t.prims_numpy(), rp)
+    assert np.array_equal(np.array(t.bounds, np.int32), rb)
+    if n == 1000:
+        with pytest.raises(ck.CukdError) as ei:
+            ck.spatial_fcp(t, torch.zeros((4, 3), dtype=torch.float32, device="cuda"))
+        assert ei.value.code == ck.ERR_UNSUPPORTED
+    ck.free(t)

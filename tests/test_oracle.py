@@ -406,3 +406,27 @@ def test_spatial_empty():
     ids, d2 = oc.spatial_fcp(oc.STACK_BASED, nodes, prims, bounds,
                              np.zeros((1, 3), np.float32), lay, datagen.uniform(4, 3, 1))
     assert np.all(ids == -1)
+
+
+def test_spatial_int_build_is_the_float_build_of_converted_coordinates():
+    """The reference builds int points through float: AtomicBox encodes
+    float(coord), the side test compares float(coord) >= pos, and Node::pos
+    / the bounds are scalar_t, so they hold the float value truncated to
+    int (spatial-kdtree.h:172-188, 427-429, 489).  With coordinates below
+    2^24 the conversion is exact, so the int tree must equal the float tree
+    over the same values in everything but the truncated pos and bounds."""
+    n = 20000
+    c = (datagen.splitmix64(5, n * 3) >> np.uint64(44)).astype(np.int32).reshape(n, 3)
+    c[:200] = c[200:400]                        # exact duplicates
+    c[400:600, 1] = 77                          # a degenerate axis
+    ni, pi, bi = oc.spatial_build(c, ck.int_points(3))
+    nf, pf, bf = oc.spatial_build(c.astype(np.float32), ck.float_points(3))
+    assert np.array_equal(pi, pf)
+    for f in ("offset", "count", "dim"):
+        assert np.array_equal(ni[f], nf[f])
+    ipos = ni.view(np.uint8).reshape(-1, 12)[:, :4].copy().view(np.int32).ravel()
+    assert np.array_equal(ipos, np.trunc(nf["pos"]).astype(np.int32))
+    assert bi.dtype == np.int32 and np.array_equal(bi, np.trunc(bf).astype(np.int32))
+    # some split lands between integers, so the truncation is exercised
+    inner = nf["count"] == 0
+    assert np.any(nf["pos"][inner] != np.trunc(nf["pos"][inner]))
