@@ -1353,6 +1353,18 @@ inline size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
 inline int cap_class(int cap) { return 31 - __builtin_clz((unsigned)cap) - 9; }  // 512 -> 0
 
+// Finish-capacity thresholds: a lone tree shrinks its capacity until it
+// has this many finish workgroups, and a forest with at least
+// CUKD_FOREST_MANY_BLOCKS at the largest capacity keeps it for every tree.
+// (The CPU emulator test builds with 1 and 1, so that small inputs reach
+// the large capacities.)
+#ifndef CUKD_FINISH_MIN_BLOCKS
+#define CUKD_FINISH_MIN_BLOCKS 256
+#endif
+#ifndef CUKD_FOREST_MANY_BLOCKS
+#define CUKD_FOREST_MANY_BLOCKS 512
+#endif
+
 // LDS-finish capacity and number of top levels of one tree.  In a forest
 // with enough finish workgroups to fill the GPU every tree takes the
 // largest capacity (fewest top levels); otherwise the capacity shrinks
@@ -1368,7 +1380,7 @@ void tree_cap(int n, bool many, bool expl, int &cap, int &L0)
   for (cap = expl ? 4096 : 8192; cap >= 512; cap >>= 1) {
     const int lg = 31 - __builtin_clz((unsigned)cap);
     L0 = h - lg > 0 ? h - lg : 0;
-    if (many || (1 << L0) >= 256 || cap == 512) break;
+    if (many || (1 << L0) >= CUKD_FINISH_MIN_BLOCKS || cap == 512) break;
   }
   if (L0 == 0) {
     // whole tree in one workgroup: capacity = 2^h
@@ -1426,7 +1438,7 @@ int make_forest_plan(ForestPlan &P, const long long *off, int ntrees, const cukd
     tree_cap(n, true, P.expl, cap[t], L0[t]);
     blocks_max_cap += 1ll << L0[t];
   }
-  const bool many = blocks_max_cap >= 512;
+  const bool many = blocks_max_cap >= CUKD_FOREST_MANY_BLOCKS;
   P.max_L0 = 0;
   for (int t = 0; t < ntrees; t++) {
     const int n = (int)(off[t + 1] - off[t]);

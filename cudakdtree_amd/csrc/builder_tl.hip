@@ -202,7 +202,7 @@ __global__ __launch_bounds__(NT) void k_tl_tile(int level, int n, int tiles_per_
                                                 int first, uint8_t *data, uint32_t *rank,
                                                 cukd_layout L)
 {
-  extern __shared__ uint32_t s_dyn[];
+  HIP_DYNAMIC_SHARED(uint32_t, s_dyn)
   __shared__ uint16_t s_ix[T];
   __shared__ uint32_t s_rank[CARRY ? T : 1];
   const int gl = blockIdx.x / tiles_per_seg;
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(NT) void k_tl_finish(int n, int L0, uint8_t *data,
                                                   const uint32_t *world, cukd_layout L)
 {
   enum { E = CAP / NT };
-  extern __shared__ uint32_t s_dyn[];  // CAP * W words of staged elements
+  HIP_DYNAMIC_SHARED(uint32_t, s_dyn)  // CAP * W words of staged elements
   __shared__ uint32_t s_xv[CAP];
   __shared__ uint32_t s_split[EXPLICIT ? CAP : 1];
   __shared__ uint8_t s_sdim[EXPLICIT ? CAP : 1];

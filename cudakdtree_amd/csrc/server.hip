@@ -5,16 +5,19 @@
 // of SERVER_SLOTS batch slots.  Each slot has a stream of its own and a
 // thread that waits for its batch and completes its requests, so batches
 // run side by side on the device: a batch held up by a few long traversals
-// does not hold up the ones admitted after it.  A heavy request (many
-// queries times k) runs in a batch of its own, so the light requests it
-// would otherwise share a batch with do not wait for it; a request larger
-// than a slot is served in pieces.  No counterpart in the reference, whose
+// does not hold up the ones admitted after it.  Inside a batch, light
+// requests are launched before heavy ones (many queries times k), one
+// launch per group, and each group's requests complete as soon as their
+// own launch has run, so a light request never waits for a heavy one it
+// shares a batch with; a request larger than a slot is served in pieces.
+// No counterpart in the reference, whose
 // queries are per-thread device calls; every result equals the single-tree
 // cukd_fcp / cukd_knn result for the same queries.
 #include <atomic>
 #include <cstdarg>
 #include <chrono>
 #include <condition_variable>
+#include <algorithm>
 #include <deque>
 #include <mutex>
 #include <string.h>
@@ -71,7 +74,7 @@ inline size_t class_bytes(size_t nq, size_t nseg, int k, int D)
 }
 
 // Work of a request, in candidate-list slots: one with at least HEAVY_COST
-// runs in a batch of its own.
+// is launched after the light requests of its batch.
 inline size_t request_cost(int nq, int k) { return (size_t)nq * (size_t)(k > 0 ? k : 1); }
 
 enum { SERVER_SLOTS = 8, HEAVY_COST = 1 << 16 };
@@ -84,9 +87,13 @@ struct Slot {
   uint8_t *dev = nullptr;   // queries | segment tables | results
   uint8_t *host = nullptr;  // pinned mirror
   size_t bytes = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t kdone = nullptr;  // the batch's queries have run
-  hipEvent_t done = nullptr;   // its results are in 'host'
+  hipStream_t stream = nullptr;  // copy-in, then per group: launch, results back
+  std::vector<hipEvent_t> cev;   // per group: its results are in 'host'
+  struct Group {
+    size_t res_off = 0, res_bytes = 0;  // the group's results in the slot
+    std::vector<int> members;           // its requests (indices into reqs)
+  };
+  std::vector<Group> groups;  // in launch order
   std::vector<Request *> reqs;
   std::vector<size_t> ids_off, d2_off;  // per request: where its results are in the slot
   size_t res_begin = 0, used = 0;  // results: [res_begin, used) of the slot
@@ -229,93 +236,132 @@ struct cukd_server {
 
 namespace {
 
+// The launch groups of a batch: one per (k, cutoff, heavy) in order of
+// first appearance, light groups before heavy ones.  Requests keep their
+// admission order inside a group.
+struct GroupKey {
+  int k;
+  float cutoff;
+  bool heavy;
+  bool same(const Request *r, bool h) const
+  {
+    return k == r->k && memcmp(&cutoff, &r->cutoff, 4) == 0 && heavy == h;
+  }
+};
+
 // Packs the requests of one batch into a slot and launches it: one H2D
-// copy, one forest query launch per (k, cutoff) class, one D2H copy of the
-// results, then an event.
+// copy, then per group one forest query launch, the copy of its results
+// back and an event, so that the slot's thread can complete the group's
+// requests as soon as their own launch has run.
 int launch_batch(cukd_server *S, Slot &sl)
 {
   const int D = S->D;
-  // classes in order of first appearance; requests keep admission order
-  // inside a class
-  struct Cls { int k; float cutoff; std::vector<int> members; };
-  std::vector<Cls> cls;
+  std::vector<GroupKey> key;
+  std::vector<std::vector<int>> mem;
   for (int i = 0; i < (int)sl.reqs.size(); i++) {
     const Request *r = sl.reqs[i];
-    int c = 0;
-    for (; c < (int)cls.size(); c++)
-      if (cls[c].k == r->k && memcmp(&cls[c].cutoff, &r->cutoff, 4) == 0) break;
-    if (c == (int)cls.size()) cls.push_back(Cls{r->k, r->cutoff, {}});
-    cls[c].members.push_back(i);
+    const bool h = request_cost(r->nq, r->k) >= (size_t)HEAVY_COST;
+    size_t g = 0;
+    for (; g < key.size(); g++)
+      if (key[g].same(r, h)) break;
+    if (g == key.size()) {
+      key.push_back(GroupKey{r->k, r->cutoff, h});
+      mem.emplace_back();
+    }
+    mem[g].push_back(i);
   }
-  // layout: per class [queries][seg_off][seg_tree]; then per class results
+  std::vector<int> order(key.size());
+  for (size_t g = 0; g < key.size(); g++) order[g] = (int)g;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return !key[x].heavy && key[y].heavy; });
+  // layout: per group [queries][seg_off][seg_tree]; then per group results
   struct Lay { size_t q, so, st, ids, d2; int nq, nseg; };
-  std::vector<Lay> lay(cls.size());
+  const size_t G = key.size();
+  std::vector<Lay> lay(G);
   size_t o = 0;
-  for (size_t c = 0; c < cls.size(); c++) {
+  for (size_t j = 0; j < G; j++) {
+    const int g = order[j];
     int nq = 0;
-    for (int i : cls[c].members) nq += sl.reqs[i]->nq;
-    lay[c].nq = nq;
-    lay[c].nseg = (int)cls[c].members.size();
-    lay[c].q = o; o = align_up(o + (size_t)nq * D * 4);
-    lay[c].so = o; o = align_up(o + (size_t)(lay[c].nseg + 1) * 4);
-    lay[c].st = o; o = align_up(o + (size_t)lay[c].nseg * 4);
+    for (int i : mem[g]) nq += sl.reqs[i]->nq;
+    lay[j].nq = nq;
+    lay[j].nseg = (int)mem[g].size();
+    lay[j].q = o; o = align_up(o + (size_t)nq * D * 4);
+    lay[j].so = o; o = align_up(o + (size_t)(lay[j].nseg + 1) * 4);
+    lay[j].st = o; o = align_up(o + (size_t)lay[j].nseg * 4);
   }
   const size_t res_begin = o;
   sl.ids_off.assign(sl.reqs.size(), 0);
   sl.d2_off.assign(sl.reqs.size(), 0);
-  for (size_t c = 0; c < cls.size(); c++) {
-    const size_t kk = cls[c].k > 0 ? (size_t)cls[c].k : 1;
-    lay[c].ids = o; o = align_up(o + (size_t)lay[c].nq * kk * 4);
-    lay[c].d2 = o; o = align_up(o + (size_t)lay[c].nq * kk * 4);
+  sl.groups.assign(G, Slot::Group());
+  for (size_t j = 0; j < G; j++) {
+    const size_t kk = key[order[j]].k > 0 ? (size_t)key[order[j]].k : 1;
+    lay[j].ids = o; o = align_up(o + (size_t)lay[j].nq * kk * 4);
+    lay[j].d2 = o; o = align_up(o + (size_t)lay[j].nq * kk * 4);
+    sl.groups[j].res_off = lay[j].ids;
+    sl.groups[j].res_bytes = o - lay[j].ids;
+    sl.groups[j].members = mem[order[j]];
   }
   if (o > sl.bytes) {
     // admission charges exactly this layout, so this is not reached
     set_last_error("batch of %zu bytes exceeds the slot of %zu", o, sl.bytes);
     return CUKD_ERR_INVALID;
   }
+  hipError_t e = hipSuccess;
+  while (sl.cev.size() < G && e == hipSuccess) {
+    hipEvent_t c = nullptr;
+    e = hipEventCreateWithFlags(&c, hipEventDisableTiming);
+    if (c) sl.cev.push_back(c);
+  }
+  if (e != hipSuccess || sl.cev.size() < G) {
+    set_last_error("batch events: %s", hipGetErrorString(e));
+    return CUKD_ERR_HIP;
+  }
   sl.res_begin = res_begin;
   sl.used = o;
   std::vector<CopyPool::Job> jobs;
-  for (size_t c = 0; c < cls.size(); c++) {
-    float *hq = (float *)(sl.host + lay[c].q);
-    int32_t *so = (int32_t *)(sl.host + lay[c].so);
-    int32_t *stt = (int32_t *)(sl.host + lay[c].st);
-    int acc = 0, s = 0;
-    const size_t kk = cls[c].k > 0 ? (size_t)cls[c].k : 1;
-    for (int i : cls[c].members) {
+  for (size_t j = 0; j < G; j++) {
+    float *hq = (float *)(sl.host + lay[j].q);
+    int32_t *so = (int32_t *)(sl.host + lay[j].so);
+    int32_t *stt = (int32_t *)(sl.host + lay[j].st);
+    int acc = 0, sg = 0;
+    const size_t kk = key[order[j]].k > 0 ? (size_t)key[order[j]].k : 1;
+    for (int i : sl.groups[j].members) {
       const Request *r = sl.reqs[i];
       jobs.push_back(CopyPool::Job{hq + (size_t)acc * D, r->q, (size_t)r->nq * D * 4});
-      so[s] = acc;
-      stt[s] = r->tree;
-      sl.ids_off[i] = lay[c].ids + (size_t)acc * kk * 4;
-      sl.d2_off[i] = lay[c].d2 + (size_t)acc * kk * 4;
+      so[sg] = acc;
+      stt[sg] = r->tree;
+      sl.ids_off[i] = lay[j].ids + (size_t)acc * kk * 4;
+      sl.d2_off[i] = lay[j].d2 + (size_t)acc * kk * 4;
       acc += r->nq;
-      s++;
+      sg++;
     }
-    so[s] = acc;
+    so[sg] = acc;
   }
   S->copies->run(jobs);
-  hipError_t e = hipMemcpyAsync(sl.dev, sl.host, res_begin, hipMemcpyHostToDevice, sl.stream);
-  for (size_t c = 0; c < cls.size() && e == hipSuccess; c++) {
-    const float *dq = (const float *)(sl.dev + lay[c].q);
-    const int32_t *dso = (const int32_t *)(sl.dev + lay[c].so);
-    const int32_t *dst = (const int32_t *)(sl.dev + lay[c].st);
-    int32_t *ids = (int32_t *)(sl.dev + lay[c].ids);
-    float *d2 = (float *)(sl.dev + lay[c].d2);
+  e = hipMemcpyAsync(sl.dev, sl.host, res_begin, hipMemcpyHostToDevice, sl.stream);
+  for (size_t j = 0; j < G && e == hipSuccess; j++) {
+    const GroupKey &gk = key[order[j]];
+    const float *dq = (const float *)(sl.dev + lay[j].q);
+    const int32_t *dso = (const int32_t *)(sl.dev + lay[j].so);
+    const int32_t *dst = (const int32_t *)(sl.dev + lay[j].st);
+    int32_t *ids = (int32_t *)(sl.dev + lay[j].ids);
+    float *d2 = (float *)(sl.dev + lay[j].d2);
     int rc;
-    if (cls[c].k == 0)
-      rc = cukd_forest_fcp(S->traversal, S->forest, S->points, S->world_bounds, dq, lay[c].nq,
-                           lay[c].nseg, dso, dst, cls[c].cutoff, 0.f, ids, d2, 0, sl.stream);
+    if (gk.k == 0)
+      rc = cukd_forest_fcp(S->traversal, S->forest, S->points, S->world_bounds, dq, lay[j].nq,
+                           lay[j].nseg, dso, dst, gk.cutoff, 0.f, ids, d2, 0, sl.stream);
     else
-      rc = cukd_forest_knn(S->traversal, CUKD_LIST_FIXED, cls[c].k, S->forest, S->points,
-                           S->world_bounds, dq, lay[c].nq, lay[c].nseg, dso, dst, cls[c].cutoff,
+      rc = cukd_forest_knn(S->traversal, CUKD_LIST_FIXED, gk.k, S->forest, S->points,
+                           S->world_bounds, dq, lay[j].nq, lay[j].nseg, dso, dst, gk.cutoff,
                            ids, d2, nullptr, 0, sl.stream);
     if (rc != CUKD_OK) return rc;
+    // the group's results come back right after its own launch, before the
+    // next group's; the slot's thread completes the group's requests once
+    // this copy is done
+    e = hipMemcpyAsync(sl.host + sl.groups[j].res_off, sl.dev + sl.groups[j].res_off,
+                       sl.groups[j].res_bytes, hipMemcpyDeviceToHost, sl.stream);
+    if (e == hipSuccess) e = hipEventRecord(sl.cev[j], sl.stream);
   }
-  // the results' copy is queued by the slot's thread once the queries have
-  // run: copies queued behind a long batch's kernels would hold up the
-  // copies of the batches launched after it
-  if (e == hipSuccess) e = hipEventRecord(sl.kdone, sl.stream);
   if (e != hipSuccess) {
     set_last_error("batch copy: %s", hipGetErrorString(e));
     return CUKD_ERR_HIP;
@@ -327,32 +373,35 @@ int launch_batch(cukd_server *S, Slot &sl)
 // its last piece.
 void complete_piece(cukd_server *S, Request *r, int status, Clock::time_point now);
 
-// Waits for a slot's batch, copies its results back and on to every
-// request's caller, and records the completions.
+// Waits for each group of a slot's batch in launch order, copies its
+// results on to its requests' callers and completes them; the slot is
+// free once every group is done.
 int finish_batch(cukd_server *S, Slot &sl)
 {
-  hipError_t e = hipEventSynchronize(sl.kdone);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(sl.host + sl.res_begin, sl.dev + sl.res_begin, sl.used - sl.res_begin,
-                       hipMemcpyDeviceToHost, sl.stream);
-  if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
-  if (e == hipSuccess) e = hipEventSynchronize(sl.done);
-  const size_t nreq = sl.reqs.size();
-  if (e == hipSuccess) {
-    std::vector<CopyPool::Job> jobs;
-    for (size_t i = 0; i < nreq; i++) {
-      const Request *r = sl.reqs[i];
-      const size_t kk = r->k > 0 ? (size_t)r->k : 1;
-      jobs.push_back(CopyPool::Job{r->ids, sl.host + sl.ids_off[i], (size_t)r->nq * kk * 4});
-      if (r->dist2)
-        jobs.push_back(CopyPool::Job{r->dist2, sl.host + sl.d2_off[i], (size_t)r->nq * kk * 4});
+  hipError_t e = hipSuccess;
+  for (size_t j = 0; j < sl.groups.size(); j++) {
+    const Slot::Group &gr = sl.groups[j];
+    if (e == hipSuccess) e = hipEventSynchronize(sl.cev[j]);
+    if (e == hipSuccess) {
+      std::vector<CopyPool::Job> jobs;
+      for (int i : gr.members) {
+        const Request *r = sl.reqs[i];
+        const size_t kk = r->k > 0 ? (size_t)r->k : 1;
+        jobs.push_back(CopyPool::Job{r->ids, sl.host + sl.ids_off[i], (size_t)r->nq * kk * 4});
+        if (r->dist2)
+          jobs.push_back(CopyPool::Job{r->dist2, sl.host + sl.d2_off[i], (size_t)r->nq * kk * 4});
+      }
+      S->copies->run(jobs);
     }
-    S->copies->run(jobs);
+    const Clock::time_point now = Clock::now();
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (int i : gr.members) {
+      complete_piece(S, sl.reqs[i], e == hipSuccess ? CUKD_OK : CUKD_ERR_HIP, now);
+      sl.reqs[i] = nullptr;
+    }
+    S->cv_done.notify_all();
   }
-  const Clock::time_point now = Clock::now();
   std::lock_guard<std::mutex> lk(S->mu);
-  for (size_t i = 0; i < nreq; i++)
-    complete_piece(S, sl.reqs[i], e == hipSuccess ? CUKD_OK : CUKD_ERR_HIP, now);
   sl.reqs.clear();
   sl.state = SLOT_FREE;
   S->cv_work.notify_one();
@@ -403,25 +452,24 @@ void worker_main(cukd_server *S)
       si = free_slot();
       Slot &sl = S->slot[si];
       // admit requests in arrival order while the batch, laid out exactly
-      // as launch_batch lays it out, fits the slot; a heavy request only
-      // as the first of a batch, which it then closes
-      struct Cls { int k; float cutoff; size_t nq, nseg, bytes; };
+      // as launch_batch lays it out (one region set per launch group),
+      // fits the slot
+      struct Cls { GroupKey key; size_t nq, nseg, bytes; };
       std::vector<Cls> cls;
       size_t bytes = 0;
       const Clock::time_point now = Clock::now();
       while (!S->pending.empty() && (S->max_per_batch <= 0 || (int)sl.reqs.size() < S->max_per_batch)) {
         Request *r = S->pending.front();
         const bool heavy = request_cost(r->nq, r->k) >= (size_t)HEAVY_COST;
-        if (heavy && !sl.reqs.empty()) break;
         size_t c = 0;
         for (; c < cls.size(); c++)
-          if (cls[c].k == r->k && memcmp(&cls[c].cutoff, &r->cutoff, 4) == 0) break;
+          if (cls[c].key.same(r, heavy)) break;
         const size_t old_b = c < cls.size() ? cls[c].bytes : 0;
         const size_t nq = (c < cls.size() ? cls[c].nq : 0) + (size_t)r->nq;
         const size_t nseg = (c < cls.size() ? cls[c].nseg : 0) + 1;
         const size_t new_b = class_bytes(nq, nseg, r->k, S->D);
         if (!sl.reqs.empty() && bytes - old_b + new_b > S->slot_bytes) break;
-        if (c == cls.size()) cls.push_back(Cls{r->k, r->cutoff, 0, 0, 0});
+        if (c == cls.size()) cls.push_back(Cls{GroupKey{r->k, r->cutoff, heavy}, 0, 0, 0});
         cls[c].nq = nq;
         cls[c].nseg = nseg;
         cls[c].bytes = new_b;
@@ -432,7 +480,6 @@ void worker_main(cukd_server *S)
         }
         sl.reqs.push_back(r);
         S->pending.pop_front();
-        if (heavy) break;
       }
       sl.state = SLOT_PACKING;
     }
@@ -523,8 +570,6 @@ int cukd_server_create(cukd_server **server, const cukd_forest *forest, const vo
     sl.bytes = S->slot_bytes;
     e = hipMalloc((void **)&sl.dev, sl.bytes);
     if (e == hipSuccess) e = hipHostMalloc((void **)&sl.host, sl.bytes, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.kdone, hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamWaitEvent(sl.stream, ready, 0);
   }
@@ -536,8 +581,6 @@ int cukd_server_create(cukd_server **server, const cukd_forest *forest, const vo
       if (sl.stream) (void)hipStreamDestroy(sl.stream);
       if (sl.dev) (void)hipFree(sl.dev);
       if (sl.host) (void)hipHostFree(sl.host);
-      if (sl.done) (void)hipEventDestroy(sl.done);
-      if (sl.kdone) (void)hipEventDestroy(sl.kdone);
     }
     delete S;
     return CUKD_ERR_ALLOC;
@@ -653,8 +696,7 @@ int cukd_server_destroy(cukd_server *S)
     (void)hipStreamDestroy(sl.stream);
     (void)hipFree(sl.dev);
     (void)hipHostFree(sl.host);
-    (void)hipEventDestroy(sl.done);
-    (void)hipEventDestroy(sl.kdone);
+    for (hipEvent_t ev : sl.cev) (void)hipEventDestroy(ev);
   }
   const int err = S->error;
   delete S;

@@ -346,8 +346,10 @@ int cukd_forest_reserve_queries(cukd_forest *forest, int32_t max_queries,
  * the batch, laid out exactly as it will be launched, fits one eighth of
  * budget_bytes of device memory, at most max_requests_per_batch requests,
  * <= 0 for no limit) and launches one forest query per (k, cutoff) class.
- * A heavy request (nq * max(k, 1) >= 65536) is batched alone, so light
- * requests never wait for it inside a batch.  A request too large for one
+ * Inside a batch, each (k, cutoff) class is launched as a light group and
+ * a heavy group (nq * max(k, 1) >= 65536), light groups first, and each
+ * group's requests complete as soon as their own launch has run, so a
+ * light request never waits for a heavy one.  A request too large for one
  * slot is queued in pieces that each fit one and completes with its last
  * piece.  Up to eight batches are in flight, each on a stream of its own,
  * and each is completed (results copied back, requests completed) as soon

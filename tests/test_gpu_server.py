@@ -195,17 +195,21 @@ def test_request_larger_than_a_slot_is_served_in_pieces(served_forest):
         assert np.array_equal(d2.view(np.uint32).ravel(), ed2.view(np.uint32).ravel())
 
 
-def test_heavy_request_batched_alone(served_forest):
-    """A heavy request (nq * k >= 65536) is not batched with the light
-    requests queued with it."""
+def test_light_and_heavy_requests_in_one_batch(served_forest):
+    """Light requests and a heavy one (nq * k >= 65536) of the same class
+    queued together run as separate launch groups of one batch, light
+    first; every result equals the single-tree query."""
     f, pts, off, lay = served_forest
     srv = ck.Server(f, pts, budget_bytes=256 << 20)
-    reqs = [(0, datagen.gmm(10, 3, 1), 4, np.inf), (3, datagen.gmm(8000, 3, 2), 16, np.inf),
-            (1, datagen.gmm(10, 3, 3), 4, np.inf), (2, datagen.gmm(20, 3, 4), 0, np.inf)]
-    _serve(srv, reqs)
-    st = srv.stats()
+    reqs = [(0, datagen.gmm(10, 3, 1), 4, np.inf), (3, datagen.gmm(20000, 3, 2), 4, np.inf),
+            (1, datagen.gmm(10, 3, 3), 4, np.inf), (2, datagen.gmm(20, 3, 4), 0, np.inf),
+            (4, datagen.gmm(9000, 3, 5), 16, np.inf)]
+    got = _serve(srv, reqs)
     srv.close()
-    assert st.requests_done == 4 and st.batches >= 2
+    for (tree, q, k, cutoff), (ids, d2) in zip(reqs, got):
+        eids, ed2 = _expected(pts, off, lay, tree, q, k, cutoff)
+        assert np.array_equal(ids.ravel(), eids.ravel())
+        assert np.array_equal(d2.view(np.uint32).ravel(), ed2.view(np.uint32).ravel())
 
 
 def test_budget_and_argument_errors(served_forest):
