@@ -447,6 +447,145 @@ __global__ __launch_bounds__(256) void k_query2(TreeView t, const float *queries
 }
 
 // ---------------------------------------------------------------------------
+// two queries per lane, pending plane distances kept in LDS
+// ---------------------------------------------------------------------------
+// The traversal above re-reads a far child's parent on pop to recompute its
+// plane distance, which costs a loop iteration of its own. Here the plane
+// distance is stored when the far child is pushed (one LDS word per depth,
+// per query), so a pop is decided before the iteration's loads: a pop
+// that passes the cull test descends into the far child in the same
+// iteration, and only a pop that fails takes one. The visits, candidate
+// updates and visit counts are exactly those of the traversal above
+// (traverse-default-stack-based.h:58-107): the stored value is the same
+// diff*diff of the same node and query, tested against the same current
+// cull.
+struct SdSlot {
+  int mode;          // SB_DESCEND, SB_POP or SB_DONE
+  int curr;          // descend: node to visit; pop: end of the path (>= n)
+  int lvl;           // level of curr
+  unsigned off;      // descend: byte offset of curr
+  unsigned pending;  // bit d: the far child at depth d of the path is still to be visited
+  float topd;        // plane distance of the deepest pending far child
+  float cull;
+};
+
+// rows of per-query LDS words the traversal needs: a far child at depth d
+// exists only if 2^d <= n, and depth d is stored in row d - 1
+__host__ __device__ inline int sd_rows(int n)
+{
+  int r = 0;
+  while (r < 31 && (2ll << r) <= (long long)n) r++;
+  return r > 0 ? r : 1;
+}
+
+// pop the deepest pending far child: descend into it if its stored plane
+// distance is still below the cull, and reload the next stored distance
+__device__ inline void sd_pop(SdSlot &s, const float *col, int row_stride, unsigned stride)
+{
+  if (s.mode != SB_POP) return;
+  const int d = 31 - __clz((int)s.pending);
+  s.pending &= ~(1u << d);
+  if (s.topd < s.cull) {
+    const int f = (((s.curr + 1) >> (s.lvl - d)) ^ 1) - 1;
+    s.curr = f;
+    s.lvl = d;
+    s.off = (unsigned)f * stride;
+    s.mode = SB_DESCEND;
+  } else if (!s.pending) {
+    s.mode = SB_DONE;
+  }
+  if (s.pending) s.topd = col[(30 - __clz((int)s.pending)) * row_stride];
+}
+
+// visit curr, push its far child (with its plane distance) if it may hold
+// closer points, and step to the close child
+template <int D, class R>
+__device__ inline void sd_visit(R &res, const float (&q)[D], const float (&p)[D], int n,
+                                unsigned stride, SdSlot &s, float *col, int row_stride,
+                                unsigned &visits)
+{
+  if (s.mode != SB_DESCEND) return;
+  visits++;
+  s.cull = res.processCandidate(s.curr, cukd::detail::sqr_dist<D>(p, q));
+  const int dim = rr_dim<D>(s.lvl);
+  const float nc = cukd::detail::pick<D>(p, dim);
+  const float qc = cukd::detail::pick<D>(q, dim);
+  const float diff = qc - nc;
+  const float dd = diff * diff;
+  const bool left_close = qc < nc;
+  const int l = 2 * s.curr + 1;
+  const int far = left_close ? l + 1 : l;
+  if (dd < s.cull && far < n) {
+    s.pending |= 2u << s.lvl;
+    col[s.lvl * row_stride] = dd;
+    s.topd = dd;
+  }
+  s.curr = left_close ? l : l + 1;
+  s.off = 2u * s.off + (left_close ? stride : 2u * stride);
+  s.lvl++;
+  if (s.curr >= n) s.mode = s.pending ? SB_POP : SB_DONE;
+}
+
+// Dynamic LDS: sd_rows(n) rows of 2 * blockDim.x floats (row r holds depth
+// r + 1 of every slot of the workgroup; each lane only touches its own two
+// columns, so no barrier is needed).
+template <int D, class R>
+__global__ __launch_bounds__(256) void k_query2s(TreeView t, const float *queries, int nq,
+                                                 float cutoff, int k, const int32_t *order,
+                                                 int32_t *ids, float *dist2, float *ret,
+                                                 ListArgs la, unsigned long long *stats)
+{
+  HIP_DYNAMIC_SHARED(float, s_pd);
+  const int nb = (int)gridDim.x;
+  const int xcd = (int)blockIdx.x & 7, q8 = nb >> 3, r8 = nb & 7;
+  const int lb = xcd * q8 + (xcd < r8 ? xcd : r8) + ((int)blockIdx.x >> 3);
+  const int i0 = lb * 2 * (int)blockDim.x + ((int)threadIdx.x >> 6) * 128 + ((int)threadIdx.x & 63);
+  if (i0 >= nq) return;
+  const bool has_b = i0 + 64 < nq;
+  const int qa = order ? order[i0] : i0;
+  const int qb = has_b ? (order ? order[i0 + 64] : i0 + 64) : qa;
+  const unsigned stride = (unsigned)t.stride;
+  const uint8_t *base = t.base + t.point_offset;
+  const int n = t.n;
+  const int row_stride = 2 * (int)blockDim.x;
+  float *col_a = s_pd + threadIdx.x;
+  float *col_b = s_pd + blockDim.x + threadIdx.x;
+  float pqa[D], pqb[D];
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    pqa[d] = queries[(size_t)qa * D + d];
+    pqb[d] = queries[(size_t)qb * D + d];
+  }
+  typename ResultOps<R>::State sta, stb;
+  R &ra = ResultOps<R>::init(sta, cutoff, k, qa, la);
+  R &rb = ResultOps<R>::init(stb, cutoff, k, qb, la);
+  SdSlot sa{n > 0 ? SB_DESCEND : SB_DONE, 0, 0, 0u, 0u, 0.f, ra.initialCullDist2()};
+  SdSlot sb{(has_b && n > 0) ? SB_DESCEND : SB_DONE, 0, 0, 0u, 0u, 0.f, rb.initialCullDist2()};
+  unsigned visits = 0;
+  while (sa.mode != SB_DONE || sb.mode != SB_DONE) {
+    sd_pop(sa, col_a, row_stride, stride);
+    sd_pop(sb, col_b, row_stride, stride);
+    // a slot that does not descend this iteration re-reads the other's
+    // node (or the root), so both loads stay in bounds and unconditional
+    const bool da = sa.mode == SB_DESCEND, db = sb.mode == SB_DESCEND;
+    const unsigned oa = da ? sa.off : (db ? sb.off : 0u);
+    const unsigned ob = db ? sb.off : oa;
+    const float *src_a = (const float *)(base + oa);
+    const float *src_b = (const float *)(base + ob);
+    float pa[D], pb[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) pa[d] = src_a[d];
+#pragma unroll
+    for (int d = 0; d < D; d++) pb[d] = src_b[d];
+    sd_visit<D>(ra, pqa, pa, n, stride, sa, col_a, row_stride, visits);
+    sd_visit<D>(rb, pqb, pb, n, stride, sb, col_b, row_stride, visits);
+  }
+  ResultOps<R>::write(sta, qa, k, ids, dist2, ret, la);
+  if (has_b) ResultOps<R>::write(stb, qb, k, ids, dist2, ret, la);
+  if (stats) atomicAdd(stats, (unsigned long long)visits);
+}
+
+// ---------------------------------------------------------------------------
 // per-dimension launchers
 // ---------------------------------------------------------------------------
 struct QueryArgs {
@@ -475,8 +614,12 @@ struct QueryArgs {
 enum { LIST_ACCUM = 100 };
 
 // gfx950 LDS per CU, and the batch size from which the two-query kernel
-// runs with fewer resident waves
-enum { LDS_PER_CU = 163840, QUERY2_FEW_WAVES_MIN_NQ = 1 << 19 };
+// runs with fewer resident waves (a build option so that small test
+// batches can take that path too)
+#ifndef CUKD_QUERY2_FEW_WAVES_MIN_NQ
+#define CUKD_QUERY2_FEW_WAVES_MIN_NQ (1 << 19)
+#endif
+enum { LDS_PER_CU = 163840, QUERY2_FEW_WAVES_MIN_NQ = CUKD_QUERY2_FEW_WAVES_MIN_NQ };
 
 template <int D, int METHOD, class R>
 inline hipError_t launch_q(const QueryArgs &a, hipStream_t s)
@@ -492,11 +635,22 @@ inline hipError_t launch_q(const QueryArgs &a, hipStream_t s)
       // waves per CU measured best for kNN-8 at 1M queries (10 % faster than
       // 32, with 2.5x less L2-miss traffic); the dynamic LDS, unused by the
       // kernel, only limits how many workgroups share a CU.
-      const size_t lds =
-          (FewWaves<R>::ok && (a.order || a.presorted) && nt == 128 &&
-           a.nq >= QUERY2_FEW_WAVES_MIN_NQ)
-              ? LDS_PER_CU / 7
-              : 0;
+      const bool few = FewWaves<R>::ok && (a.order || a.presorted) && nt == 128 &&
+                       a.nq >= QUERY2_FEW_WAVES_MIN_NQ;
+      const size_t lds = few ? LDS_PER_CU / 7 : 0;
+#ifndef CUKD_NO_STORED_PLANES
+      // the same launch shape with the pending plane distances in the LDS
+      // that the cap reserves (a few more rows for deep trees)
+      if constexpr (FewWaves<R>::ok) {
+        if (few && !a.fq.seg_off) {
+          const size_t need = (size_t)sd_rows(a.t.n) * 2 * nt * sizeof(float);
+          hipLaunchKernelGGL((k_query2s<D, R>), dim3(blocks), dim3(nt), need > lds ? need : lds,
+                             s, a.t, a.queries, a.nq, a.cutoff, a.k, a.order, a.ids, a.dist2,
+                             a.ret, a.la, a.stats);
+          return hipGetLastError();
+        }
+      }
+#endif
       if (a.fq.seg_off)
         hipLaunchKernelGGL((k_query2<D, R, true>), dim3(blocks), dim3(nt), lds, s, a.t, a.queries,
                            a.nq, a.cutoff, a.k, a.order, a.ids, a.dist2, a.ret, a.la, a.fq, a.stats);

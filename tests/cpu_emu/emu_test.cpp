@@ -51,6 +51,8 @@ std::vector<uint32_t> points(int n, const cukd_layout &L, uint64_t seed, const c
       const uint64_t r = sm(seed);
       float f;
       if (!strcmp(dist, "ties")) f = (float)(r % 7);
+      else if (!strcmp(dist, "dups")) f = 0.5f;
+      else if (!strcmp(dist, "twovals")) f = (float)(r & 1);
       else if (!strcmp(dist, "coarse")) f = (float)(r % 4096) / 4096.f;
       else f = (float)(r >> 40) * (1.f / 16777216.f);
       uint32_t b;
@@ -266,6 +268,14 @@ int main(int argc, char **argv)
   for (int n : {3000, 20000})
     cases.push_back({"build default float3 ties n=" + std::to_string(n),
                      [=] { return build_case("ties", n, f3, CUKD_BUILDER_ID_DEFAULT, 13 + n); }});
+  // tie groups that span whole subtrees: every key word tied down to the
+  // index (dups), or few distinct points (twovals)
+  cases.push_back({"build default float3 dups n=20000",
+                   [=] { return build_case("dups", 20000, f3, CUKD_BUILDER_ID_DEFAULT, 14); }});
+  cases.push_back({"build default float3 twovals n=20000",
+                   [=] { return build_case("twovals", 20000, f3, CUKD_BUILDER_ID_DEFAULT, 15); }});
+  cases.push_back({"build explicit-dim payload dups n=9000",
+                   [=] { return build_case("dups", 9000, ppp, CUKD_BUILDER_ID_DEFAULT, 16); }});
   cases.push_back({"build default float2 coarse n=30000",
                    [=] { return build_case("coarse", 30000, f2, CUKD_BUILDER_ID_DEFAULT, 17); }});
   cases.push_back({"build default int4 ties n=12000",
@@ -286,6 +296,18 @@ int main(int argc, char **argv)
   cases.push_back({"query kNN-8 Morton-sorted", [=] {
                      return query_case(6000, 3000, 3, CUKD_STACK_BASED, 8, CUKD_QUERY_SORT, 47);
                    }});
+  // sorted batches of >= 1024 queries run the large-batch kernel in the
+  // bigcap build: deep and shallow trees, k = 4 and 16, 2 and 4 dims
+  cases.push_back({"query kNN-4 float2 sorted n=4097", [=] {
+                     return query_case(4097, 2500, 2, CUKD_STACK_BASED, 4, CUKD_QUERY_SORT, 73);
+                   }});
+  cases.push_back({"query kNN-16 float4 sorted n=3000", [=] {
+                     return query_case(3000, 1500, 4, CUKD_STACK_BASED, 16, CUKD_QUERY_SORT, 79);
+                   }});
+  for (int n : {1, 2, 3, 64})
+    cases.push_back({"query kNN-8 sorted tiny n=" + std::to_string(n), [=] {
+                       return query_case(n, 1100, 3, CUKD_STACK_BASED, 8, CUKD_QUERY_SORT, 83 + n);
+                     }});
   cases.push_back({"query kNN-4 float2 one query per lane", [=] {
                      return query_case(5000, 2000, 2, CUKD_STACK_BASED, 4, CUKD_QUERY_ONE_PER_LANE, 53);
                    }});

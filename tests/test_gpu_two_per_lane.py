@@ -143,7 +143,7 @@ def test_large_sorted_and_presorted_launches_agree():
     dq = torch.from_numpy(q).cuda()
     order = ck.query_order(dq).long()
     qs = dq[order].contiguous()
-    for k in (1, 8):
+    for k in (1, 4, 8, 16):
         plain = ck.knn(dt, lay, qs, k)
         pres = ck.knn(dt, lay, qs, k, flags=ck.QUERY_PRESORTED)
         srt = ck.knn(dt, lay, dq, k, flags=ck.QUERY_SORT)
@@ -158,3 +158,44 @@ def test_large_sorted_and_presorted_launches_agree():
     sub = qs[:20000].cpu().numpy()
     check_knn(*s, sub, 8, float("inf"), lists=(ck.LIST_FIXED,), methods=SB,
               flags=ck.QUERY_PRESORTED)
+
+
+@pytest.mark.parametrize("dist", ["coarse", "ties"])
+def test_large_sorted_launch_with_ties(dist):
+    """Tie-heavy points and queries at a large presorted batch (the launch
+    that keeps pending plane distances in LDS): equal to the one-query-per-
+    lane kernel, which the oracle pins on tied input elsewhere, and to the
+    oracle on a presorted slice."""
+    s = _setup(distribution(dist, 300_000, 3, seed=57))
+    lay, tree, box, dt, db = s
+    q = distribution(dist, 1 << 19, 3, seed=58) + np.float32(0.25)
+    dq = torch.from_numpy(np.ascontiguousarray(q, dtype=np.float32)).cuda()
+    qs = dq[ck.query_order(dq).long()].contiguous()
+    for k in (4, 8):
+        pres = ck.knn(dt, lay, qs, k, flags=ck.QUERY_PRESORTED)
+        one = ck.knn(dt, lay, qs, k, flags=ONE)
+        torch.cuda.synchronize()
+        for a, b in zip(pres, one):
+            assert torch.equal(a, b), k
+    check_knn(*s, qs[:5000].cpu().numpy(), 8, float("inf"), lists=(ck.LIST_FIXED,),
+              methods=SB, flags=ck.QUERY_PRESORTED)
+
+
+def test_large_sorted_launch_statistics_exact():
+    """The large presorted launch counts exactly the visits of the
+    one-query-per-lane traversal (the reference's CUKD_STATS count)."""
+    s = _setup(distribution("uniform", 400_000, 3, seed=61))
+    lay, tree, box, dt, db = s
+    dq = torch.from_numpy(datagen.uniform(1 << 19, 3, 62)).cuda()
+    qs = dq[ck.query_order(dq).long()].contiguous()
+    counts = []
+    for flags in (ck.QUERY_PRESORTED, ONE):
+        counter = torch.zeros(1, dtype=torch.int64, device="cuda")
+        ck.set_traversal_stats(counter)
+        try:
+            ck.knn(dt, lay, qs, 8, flags=flags)
+            torch.cuda.synchronize()
+        finally:
+            ck.set_traversal_stats(None)
+        counts.append(int(counter.item()))
+    assert counts[0] == counts[1] > 0
