@@ -57,6 +57,7 @@ struct SegState {
   uint32_t prefix;    // high bits of 'word' fixed so far
   uint32_t resolved;  // words[0..word] identify the pivot uniquely
   uint32_t tiles_done;
+  uint32_t gcount;    // tie-group elements gathered on this level
   uint32_t dim;       // split dim of this subtree
   uint32_t chain;     // split dims, most recent first, 4 bits each
   uint32_t chain_len; // number of coordinate words in the key
@@ -261,7 +262,7 @@ __global__ __launch_bounds__(256) void k_copy_in(TreeSet<InTree> set, uint8_t *d
 // Per-level reset of the subtree states; round-robin dims and chains are
 // closed forms of the level.  One thread per segment of the level.
 __global__ void k_level_init(TreeSet<LvlTree> set, int nsegs, SegState *st,
-                             uint32_t *cursors, uint32_t *need, int level, int D,
+                             uint32_t *cursors, uint32_t *need, uint32_t *tb, int level, int D,
                              int explicit_dim)
 {
   const int sg = blockIdx.x * blockDim.x + threadIdx.x;
@@ -277,6 +278,7 @@ __global__ void k_level_init(TreeSet<LvlTree> set, int nsegs, SegState *st,
   t.prefix = 0;
   t.resolved = 0;
   t.tiles_done = 0;
+  t.gcount = 0;
   if (!explicit_dim) {
     t.dim = (uint32_t)(level % D);
     uint32_t chain = 0, len = 0;
@@ -286,6 +288,10 @@ __global__ void k_level_init(TreeSet<LvlTree> set, int nsegs, SegState *st,
   }
   cursors[2 * sg] = 0u;
   cursors[2 * sg + 1] = 0u;
+  for (int w = 0; w < MAX_WORDS; w++) {
+    tb[(size_t)sg * 2 * MAX_WORDS + w] = 0xffffffffu;
+    tb[(size_t)sg * 2 * MAX_WORDS + MAX_WORDS + w] = 0u;
+  }
 }
 
 // Value of composite-key word w of an element (word 0 is the cached key).
@@ -307,6 +313,44 @@ __device__ inline int tiles_of(const LevelGeom &g, int s, int T)
   const int a = g.offset(s);
   const int b = a + g.size(s) - 1;
   return b / T - a / T + 1;
+}
+
+// The bin of h[0..nbins) that holds rank krem, into s_pick as (bin, count
+// of lower bins, count of the bin); every thread of the workgroup calls it.
+// Per-thread sums of consecutive bins, an inclusive scan over the threads,
+// then the owning thread walks its own few bins (a serial walk over all
+// bins would cost a dependent LDS load per bin).  nbins is a multiple of
+// SELECT_NT.
+__device__ inline void pick_bin(const uint32_t *h, int nbins, uint32_t krem, uint32_t *s_part,
+                                uint32_t *s_pick)
+{
+  const int per = nbins / SELECT_NT;
+  const int b0 = (int)threadIdx.x * per;
+  uint32_t mine = 0;
+  for (int i = 0; i < per; i++) mine += h[b0 + i];
+  s_part[threadIdx.x] = mine;
+  __syncthreads();
+  for (int off = 1; off < SELECT_NT; off <<= 1) {
+    const uint32_t add = (int)threadIdx.x >= off ? s_part[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s_part[threadIdx.x] += add;
+    __syncthreads();
+  }
+  const uint32_t incl = s_part[threadIdx.x];
+  if (krem >= incl - mine && krem < incl) {
+    uint32_t before = incl - mine, cnt = 0;
+    int b = b0;
+    for (int i = 0; i < per; i++) {
+      cnt = h[b0 + i];
+      b = b0 + i;
+      if (before + cnt > krem) break;
+      before += cnt;
+    }
+    s_pick[0] = (uint32_t)b;
+    s_pick[1] = before;
+    s_pick[2] = cnt;
+  }
+  __syncthreads();
 }
 
 // One digit pass of the radix select.  Every tile histograms the digit of
@@ -469,40 +513,8 @@ __global__ __launch_bounds__(SELECT_NT) void k_select_pass(
       s_hist[sl][b] = sum;
     }
     __syncthreads();
-    if (!s_st[sl].resolved && (int)s_st[sl].word == word) {
-      // the bin holding rank krem: per-thread sums of consecutive bins, an
-      // inclusive scan over the threads, then the owning thread walks its
-      // own few bins (a serial walk over all bins would cost a dependent
-      // LDS load per bin)
-      const int per = nbins / SELECT_NT;
-      const int b0 = (int)threadIdx.x * per;
-      uint32_t mine = 0;
-      for (int i = 0; i < per; i++) mine += s_hist[sl][b0 + i];
-      s_part[threadIdx.x] = mine;
-      __syncthreads();
-      for (int off = 1; off < SELECT_NT; off <<= 1) {
-        const uint32_t add = (int)threadIdx.x >= off ? s_part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        s_part[threadIdx.x] += add;
-        __syncthreads();
-      }
-      const uint32_t incl = s_part[threadIdx.x];
-      const uint32_t krem = s_st[sl].krem;
-      if (krem >= incl - mine && krem < incl) {
-        uint32_t before = incl - mine, cnt = 0;
-        int b = b0;
-        for (int i = 0; i < per; i++) {
-          cnt = s_hist[sl][b0 + i];
-          b = b0 + i;
-          if (before + cnt > krem) break;
-          before += cnt;
-        }
-        s_pick[0] = (uint32_t)b;
-        s_pick[1] = before;
-        s_pick[2] = cnt;
-      }
-      __syncthreads();
-    }
+    if (!s_st[sl].resolved && (int)s_st[sl].word == word)
+      pick_bin(s_hist[sl], nbins, s_st[sl].krem, s_part, s_pick);
     if (threadIdx.x == 0) {
       SegState t = s_st[sl];
       t.tiles_done = 0;
@@ -538,6 +550,264 @@ __global__ __launch_bounds__(SELECT_NT) void k_select_pass(
           }
         }
       }
+      st[s] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// Tied pivots of a level, in one launch.  After the word-0 passes, a
+// subtree that is not resolved holds several elements whose key word 0
+// equals its pivot's.  Its pivot is the one of rank krem among them in the
+// order of the remaining key words: the split-dim chain's earlier
+// coordinates, then the input index, which is the order the host builder's
+// stable sorts leave (builder_thrust.h:126-142).
+//
+// Every tile finds its tie-group elements, reserves room for them with one
+// atomic per subtree and round, and stores their remaining key words in
+// planes (plane j holds word j + 1 of every member).  The planes live in
+// the subtree's own range of a buffer that the level leaves unused (the
+// next level's element array, which has at least as many words per element
+// as there are tie words), so a group of any size fits.  The tile that
+// completes the subtree (the counter protocol of k_select_pass) then runs
+// the digit passes over those planes alone, inside its workgroup.  The
+// gather also keeps each tie word's smallest and largest value over the
+// group (tb, per subtree of the level): a word on which the whole group
+// agrees is fixed without a pass, and so is every leading digit that its
+// bounds share (all-equal points thus cost the index word's low digits
+// only).  With no tied subtree on the level the launch returns at once.
+enum { TIE_BATCH = 8 };
+__global__ __launch_bounds__(SELECT_NT) void k_tie_resolve(
+    TreeSet<LvlTree> set, int level, const uint32_t *key, const uint8_t *data,
+    const uint32_t *idx, SegState *st, uint8_t *scratch, const uint32_t *need, uint32_t *tb,
+    cukd_layout L)
+{
+  if (__hip_atomic_load(&need[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
+  enum { NW = SELECT_NT / 64 };
+  __shared__ uint32_t s_hist[DIGIT_MAX_BINS];
+  __shared__ SegState s_st[2];
+  __shared__ int s_last[2];
+  __shared__ uint32_t s_part[SELECT_NT];
+  __shared__ uint32_t s_pick[3];
+  __shared__ uint32_t s_wcnt[2][NW];  // members of one round per (slot, wave), then their offsets
+  __shared__ uint32_t s_tmin[2][MAX_WORDS], s_tmax[2][MAX_WORDS];  // tie-word bounds of the tile
+  __shared__ LvlTree s_tree;
+  if (threadIdx.x == 0)
+    s_tree = find_tree(set, (int)blockIdx.x, [](const LvlTree &x) { return x.stile0; });
+  __syncthreads();
+  const LvlTree tr = s_tree;
+  LevelGeom g;
+  g.init(tr.n, level);
+  const int T = tr.tsel;
+  const int W = L.stride >> 2;
+  key += tr.base;
+  data += (size_t)tr.base * L.stride;
+  idx += tr.base;
+  uint32_t *words = (uint32_t *)(scratch + (size_t)tr.base * L.stride);
+  st += tr.seg;
+  const bool isf = L.scalar_type == CUKD_SCALAR_F32;
+  const int t0 = ((int)blockIdx.x - tr.stile0) * T;
+  const int t1 = min(t0 + T, g.n_elems);
+  const int seg0 = g.seg_of(t0);
+  const int seg1 = g.seg_of(t1 - 1);
+  if (threadIdx.x == 0) s_st[0] = st[seg0];
+  if (threadIdx.x == 1) s_st[1] = st[seg1];
+  if (threadIdx.x < 2 * MAX_WORDS) {
+    (&s_tmin[0][0])[threadIdx.x] = 0xffffffffu;
+    (&s_tmax[0][0])[threadIdx.x] = 0u;
+  }
+  __syncthreads();
+  // tiles inside one subtree use slot 0 only
+  const int nslots = seg1 != seg0 ? 2 : 1;
+  const bool tied0 = !s_st[0].resolved;
+  const bool tied1 = nslots == 2 && !s_st[1].resolved;
+  if (!tied0 && !tied1) return;
+  const int seg1_begin = nslots == 2 ? g.offset(seg1) : g.n_elems;
+  const int twmax = max((int)s_st[0].chain_len, nslots == 2 ? (int)s_st[1].chain_len : 0);
+  const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+  const uint64_t below = (1ull << lane) - 1ull;
+  // gather
+  for (int base = t0; base < t1; base += SELECT_NT * TIE_BATCH) {
+    uint32_t kv[TIE_BATCH];
+#pragma unroll
+    for (int u = 0; u < TIE_BATCH; u++) {
+      const int p = base + u * SELECT_NT + (int)threadIdx.x;
+      const int slot = p >= seg1_begin ? 1 : 0;
+      kv[u] = p < t1 && (slot ? tied1 : tied0) ? key[p] : 0u;
+    }
+    bool in[TIE_BATCH];
+    uint32_t rank[TIE_BATCH];
+    uint32_t wcnt[2] = {0u, 0u};
+#pragma unroll
+    for (int u = 0; u < TIE_BATCH; u++) {
+      const int p = base + u * SELECT_NT + (int)threadIdx.x;
+      const int slot = p >= seg1_begin ? 1 : 0;
+      in[u] = p < t1 && (slot ? tied1 : tied0) && kv[u] == s_st[slot].fixed[0];
+      rank[u] = 0u;
+      for (int sl = 0; sl < nslots; sl++) {
+        const uint64_t m = __ballot(in[u] && slot == sl);
+        if (in[u] && slot == sl) rank[u] = wcnt[sl] + (uint32_t)__popcll(m & below);
+        wcnt[sl] += (uint32_t)__popcll(m);
+      }
+    }
+    if (lane == 0) {
+      s_wcnt[0][wave] = wcnt[0];
+      s_wcnt[1][wave] = wcnt[1];
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 && (int)threadIdx.x < nslots) {
+      // one reservation per subtree and round; the waves' offsets follow
+      const int sl = (int)threadIdx.x;
+      uint32_t tot = 0u;
+      for (int w = 0; w < NW; w++) {
+        const uint32_t c = s_wcnt[sl][w];
+        s_wcnt[sl][w] = tot;
+        tot += c;
+      }
+      const uint32_t b = tot ? atomicAdd(&st[sl ? seg1 : seg0].gcount, tot) : 0u;
+      for (int w = 0; w < NW; w++) s_wcnt[sl][w] += b;
+    }
+    __syncthreads();
+    // the members' tie words into the planes and the tile's word bounds
+    // (one LDS update per wave where the wave's members agree)
+#pragma unroll
+    for (int u = 0; u < TIE_BATCH; u++) {
+      const int p = base + u * SELECT_NT + (int)threadIdx.x;
+      const int slot = p >= seg1_begin ? 1 : 0;
+      const SegState &t = s_st[slot];
+      const int s = slot ? seg1 : seg0;
+      const uint32_t j = in[u] ? s_wcnt[slot][wave] + rank[u] : 0u;
+      const int size = g.size(s);
+      uint32_t *reg = words + (size_t)g.offset(s) * W;
+      const uint8_t *elem = data + (size_t)(in[u] ? p : t0) * L.stride;
+      const uint32_t id = in[u] ? idx[p] : 0u;
+      for (int w = 1; w <= twmax; w++) {
+        const bool mine = in[u] && w <= (int)t.chain_len;
+        const uint32_t v = mine ? key_word(w, t, kv[u], elem, id, L, isf) : 0u;
+        if (mine) reg[(size_t)(w - 1) * size + j] = v;
+        for (int sl = 0; sl < nslots; sl++) {
+          const bool c = mine && slot == sl;
+          const uint64_t act = __ballot(c);
+          if (!act) continue;
+          const int first = __ffsll((unsigned long long)act) - 1;
+          const uint32_t fv = __shfl(v, first);
+          if (__ballot(c && v == fv) == act) {
+            if (lane == first) {
+              atomicMin(&s_tmin[sl][w], fv);
+              atomicMax(&s_tmax[sl][w], fv);
+            }
+          } else if (c) {
+            atomicMin(&s_tmin[sl][w], v);
+            atomicMax(&s_tmax[sl][w], v);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // the tile's word bounds into the subtree's
+  if ((int)threadIdx.x < nslots * MAX_WORDS) {
+    const int sl = (int)threadIdx.x / MAX_WORDS, w = (int)threadIdx.x % MAX_WORDS;
+    uint32_t *b = tb + (size_t)(tr.hseg + (sl ? seg1 : seg0)) * 2 * MAX_WORDS;
+    if (s_tmin[sl][w] <= s_tmax[sl][w]) {
+      atomicMin(&b[w], s_tmin[sl][w]);
+      atomicMax(&b[MAX_WORDS + w], s_tmax[sl][w]);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int sl = 0; sl < 2; sl++) {
+      s_last[sl] = 0;
+      if (sl < nslots && (sl ? tied1 : tied0)) {
+        const int s = sl ? seg1 : seg0;
+        // release: this tile's planes happen-before the last tile's reads
+        const uint32_t done = __hip_atomic_fetch_add(&st[s].tiles_done, 1u, __ATOMIC_RELEASE,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+        s_last[sl] = (int)done == tiles_of(g, s, T) - 1;
+      }
+    }
+  }
+  __syncthreads();
+  for (int sl = 0; sl < nslots; sl++) {
+    if (!s_last[sl]) continue;
+    // every thread acquires before reading the other tiles' planes
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int s = sl ? seg1 : seg0;
+    const uint32_t m = __hip_atomic_load(&st[s].gcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const size_t size = (size_t)g.size(s);
+    const uint32_t *reg = words + (size_t)g.offset(s) * W;
+    const uint32_t *bnd = tb + (size_t)(tr.hseg + s) * 2 * MAX_WORDS;
+    SegState t = s_st[sl];
+    for (int w = (int)t.word; w <= (int)t.chain_len && !t.resolved; w++) {
+      const uint32_t *plane = reg + (size_t)(w - 1) * size;
+      const uint32_t lo = __hip_atomic_load(&bnd[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t hi =
+          __hip_atomic_load(&bnd[MAX_WORDS + w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t prefix = 0u, cnt = 0u;
+      if (lo == hi) {
+        // the whole group agrees on this word: still tied (cnt >= 2)
+        t.fixed[w] = lo;
+        t.word = (uint32_t)w;
+        continue;
+      }
+      for (int d = 0; d < 3; d++) {
+        const int shift = d == 0 ? 21 : (d == 1 ? 10 : 0);
+        const int bits = d == 2 ? 10 : 11;
+        const int nbins = 1 << bits;
+        const uint32_t mask = (uint32_t)nbins - 1u;
+        const int hi_shift = shift + bits;
+        if ((lo >> shift) == (hi >> shift)) {
+          // every candidate is in one bin: krem and the candidates stay
+          // (never the last digit, since lo != hi)
+          prefix = (hi_shift < 32 ? (prefix << bits) : 0u) | ((lo >> shift) & mask);
+          continue;
+        }
+        for (int i = threadIdx.x; i < nbins; i += blockDim.x) s_hist[i] = 0u;
+        __syncthreads();
+        for (uint32_t i0 = 0; i0 < m; i0 += SELECT_NT * TIE_BATCH) {
+          uint32_t v[TIE_BATCH];
+#pragma unroll
+          for (int u = 0; u < TIE_BATCH; u++) {
+            const uint32_t i = i0 + u * SELECT_NT + threadIdx.x;
+            v[u] = i < m ? plane[i] : 0u;
+          }
+#pragma unroll
+          for (int u = 0; u < TIE_BATCH; u++) {
+            const uint32_t i = i0 + u * SELECT_NT + threadIdx.x;
+            bool has = i < m;
+            // candidates agree with the pivot on the earlier tie words
+            for (int w2 = 1; w2 < w && has; w2++)
+              has = reg[(size_t)(w2 - 1) * size + i] == t.fixed[w2];
+            if (has && hi_shift < 32 && (v[u] >> hi_shift) != prefix) has = false;
+            const uint32_t hb = (v[u] >> shift) & mask;
+            const uint64_t act = __ballot(has);
+            if (act) {
+              const int first = __ffsll((unsigned long long)act) - 1;
+              const uint32_t fb = __shfl(hb, first);
+              const uint64_t same = __ballot(has && hb == fb);
+              if (same == act) {
+                if (lane == first) atomicAdd(&s_hist[fb], (uint32_t)__popcll(act));
+              } else if (has) {
+                atomicAdd(&s_hist[hb], 1u);
+              }
+            }
+          }
+        }
+        __syncthreads();
+        pick_bin(s_hist, nbins, t.krem, s_part, s_pick);
+        t.krem -= s_pick[1];
+        cnt = s_pick[2];
+        prefix = (hi_shift < 32 ? (prefix << bits) : 0u) | s_pick[0];
+      }
+      t.fixed[w] = prefix;
+      t.word = (uint32_t)w;
+      if (cnt == 1u) t.resolved = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      t.tiles_done = 0;
+      t.gcount = 0;
+      t.prefix = 0;
       st[s] = t;
     }
     __syncthreads();
@@ -1428,6 +1698,8 @@ int make_forest_plan(ForestPlan &P, const long long *off, int ntrees, const cukd
   P.expl = L.dim_offset >= 0;
   P.ntrees = ntrees;
   P.ntotal = off[ntrees] - off[0];
+  for (int t = 0; t < ntrees; t++)
+    if (off[t + 1] - off[t] > P.max_n) P.max_n = off[t + 1] - off[t];
   P.tables = device_tables;
   const int D = L.num_dims;
   std::vector<int> cap(ntrees, 0), L0(ntrees, 0);
@@ -1538,6 +1810,7 @@ int make_forest_plan(ForestPlan &P, const long long *off, int ntrees, const cukd
   P.off_hist = o; o = align_up(o + P.max_hseg * HIST_STRIDE * 4);
   P.off_cursors = o; o = align_up(o + P.max_hseg * 2 * 4);
   P.off_need = o; o = align_up(o + MAX_WORDS * 4);
+  P.off_tb = o; o = align_up(o + P.max_hseg * 2 * MAX_WORDS * 4);
   P.zero_end = o;
   P.off_acc = o; o = align_up(o + (size_t)ntrees * 2 * D * 4);
   if (device_tables) {
@@ -1599,6 +1872,7 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
   uint32_t *hist = (uint32_t *)(ws + P.off_hist);
   uint32_t *cursors = (uint32_t *)(ws + P.off_cursors);
   uint32_t *need = (uint32_t *)(ws + P.off_need);
+  uint32_t *tb = (uint32_t *)(ws + P.off_tb);
   uint32_t *acc = (uint32_t *)(ws + P.off_acc);
   uint8_t *pts = (uint8_t *)d_points;
 
@@ -1625,7 +1899,7 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
     const TreeSet<LvlTree> set =
         make_set(P, ws, P.tables ? P.off_tab_lvl[level] : 0, P.lvl[level]);
     hipLaunchKernelGGL(k_level_init, dim3((P.lvl_segs[level] + 255) / 256), dim3(256), 0, s, set,
-                       P.lvl_segs[level], st, cursors, need, level, D, expl ? 1 : 0);
+                       P.lvl_segs[level], st, cursors, need, tb, level, D, expl ? 1 : 0);
     const int max_words = expl ? D + 1 : (level + 1 < D ? level + 1 : D) + 1;
     // histogram copies per subtree where many tiles add into one subtree
     // (at least 64 tiles per copy: the last tile reads every copy, and 32 or
@@ -1636,15 +1910,19 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
     while (hcopies < SEL_HCOPIES && tiles_per_seg >= 64 * hcopies &&
            (size_t)P.lvl_segs[level] * hcopies * 2 <= P.max_hseg)
       hcopies *= 2;
-    for (int w = 0; w < max_words; w++) {
-      const int shifts[3] = {21, 10, 0};
-      const int bitsv[3] = {11, 11, 10};
-      for (int d = 0; d < 3; d++)
-        hipLaunchKernelGGL(k_select_pass, dim3(P.lvl_stiles[level]), dim3(SELECT_NT), 0, s, set,
-                           level, w, shifts[d], bitsv[d], (const uint32_t *)kcur,
-                           (const uint8_t *)dcur, (const uint32_t *)icur, st, hist, need, L,
-                           expl ? 1 : 0, hcopies);
-    }
+    // key word 0 (the split coordinate) over the whole level, then the
+    // few tied subtrees over their tie groups alone, in one launch
+    const int shifts[3] = {21, 10, 0};
+    const int bitsv[3] = {11, 11, 10};
+    for (int d = 0; d < 3; d++)
+      hipLaunchKernelGGL(k_select_pass, dim3(P.lvl_stiles[level]), dim3(SELECT_NT), 0, s, set,
+                         level, 0, shifts[d], bitsv[d], (const uint32_t *)kcur,
+                         (const uint8_t *)dcur, (const uint32_t *)icur, st, hist, need, L,
+                         expl ? 1 : 0, hcopies);
+    if (max_words > 1)
+      hipLaunchKernelGGL(k_tie_resolve, dim3(P.lvl_stiles[level]), dim3(SELECT_NT), 0, s, set,
+                         level, (const uint32_t *)kcur, (const uint8_t *)dcur,
+                         (const uint32_t *)icur, st, dnext, (const uint32_t *)need, tb, L);
     const int W = L.stride >> 2;
     auto kp = W <= 3 ? k_partition<3> : (W == 4 ? k_partition<4> : k_partition<0>);
     hipLaunchKernelGGL(kp, dim3(P.lvl_ptiles[level]), dim3(SEL_NT), 0, s, set, level,

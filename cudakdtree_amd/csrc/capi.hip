@@ -913,6 +913,7 @@ static int forest_query(int traversal, int list_kind, int k, const cukd_forest *
   a.la = ListArgs{nullptr, 0, 0};
   a.fq = ForestQ{d_seg_offsets, d_seg_tree, num_segments, forest->tree_off};
   a.tree_bytes = (uint64_t)forest->plan.ntotal * (uint64_t)L.stride;
+  a.max_n = (int)forest->plan.max_n;
   a.stats = g_stats;
   MemResource mr{forest->has_mem ? &forest->mem : nullptr};
   // the reserved ordering scratch serves queries on its own stream only;

@@ -529,11 +529,12 @@ __device__ inline void sd_visit(R &res, const float (&q)[D], const float (&p)[D]
 // Dynamic LDS: sd_rows(n) rows of 2 * blockDim.x floats (row r holds depth
 // r + 1 of every slot of the workgroup; each lane only touches its own two
 // columns, so no barrier is needed).
-template <int D, class R>
+template <int D, class R, bool FOREST>
 __global__ __launch_bounds__(256) void k_query2s(TreeView t, const float *queries, int nq,
                                                  float cutoff, int k, const int32_t *order,
                                                  int32_t *ids, float *dist2, float *ret,
-                                                 ListArgs la, unsigned long long *stats)
+                                                 ListArgs la, ForestQ fq,
+                                                 unsigned long long *stats)
 {
   HIP_DYNAMIC_SHARED(float, s_pd);
   const int nb = (int)gridDim.x;
@@ -546,7 +547,17 @@ __global__ __launch_bounds__(256) void k_query2s(TreeView t, const float *querie
   const int qb = has_b ? (order ? order[i0 + 64] : i0 + 64) : qa;
   const unsigned stride = (unsigned)t.stride;
   const uint8_t *base = t.base + t.point_offset;
-  const int n = t.n;
+  unsigned tb_a = 0, tb_b = 0;  // byte offset of the slot's tree in the forest
+  int n_a = t.n, n_b = t.n;
+  if constexpr (FOREST) {
+    const int ta = fq.seg_tree[segment_of(fq.seg_off, fq.nseg, qa)];
+    const int tb = fq.seg_tree[segment_of(fq.seg_off, fq.nseg, qb)];
+    const int oa = fq.tree_off[ta], ob = fq.tree_off[tb];
+    tb_a = (unsigned)oa * stride;
+    tb_b = (unsigned)ob * stride;
+    n_a = fq.tree_off[ta + 1] - oa;
+    n_b = fq.tree_off[tb + 1] - ob;
+  }
   const int row_stride = 2 * (int)blockDim.x;
   float *col_a = s_pd + threadIdx.x;
   float *col_b = s_pd + blockDim.x + threadIdx.x;
@@ -559,8 +570,8 @@ __global__ __launch_bounds__(256) void k_query2s(TreeView t, const float *querie
   typename ResultOps<R>::State sta, stb;
   R &ra = ResultOps<R>::init(sta, cutoff, k, qa, la);
   R &rb = ResultOps<R>::init(stb, cutoff, k, qb, la);
-  SdSlot sa{n > 0 ? SB_DESCEND : SB_DONE, 0, 0, 0u, 0u, 0.f, ra.initialCullDist2()};
-  SdSlot sb{(has_b && n > 0) ? SB_DESCEND : SB_DONE, 0, 0, 0u, 0u, 0.f, rb.initialCullDist2()};
+  SdSlot sa{n_a > 0 ? SB_DESCEND : SB_DONE, 0, 0, 0u, 0u, 0.f, ra.initialCullDist2()};
+  SdSlot sb{(has_b && n_b > 0) ? SB_DESCEND : SB_DONE, 0, 0, 0u, 0u, 0.f, rb.initialCullDist2()};
   unsigned visits = 0;
   while (sa.mode != SB_DONE || sb.mode != SB_DONE) {
     sd_pop(sa, col_a, row_stride, stride);
@@ -568,8 +579,8 @@ __global__ __launch_bounds__(256) void k_query2s(TreeView t, const float *querie
     // a slot that does not descend this iteration re-reads the other's
     // node (or the root), so both loads stay in bounds and unconditional
     const bool da = sa.mode == SB_DESCEND, db = sb.mode == SB_DESCEND;
-    const unsigned oa = da ? sa.off : (db ? sb.off : 0u);
-    const unsigned ob = db ? sb.off : oa;
+    const unsigned oa = da ? tb_a + sa.off : (db ? tb_b + sb.off : 0u);
+    const unsigned ob = db ? tb_b + sb.off : oa;
     const float *src_a = (const float *)(base + oa);
     const float *src_b = (const float *)(base + ob);
     float pa[D], pb[D];
@@ -577,8 +588,8 @@ __global__ __launch_bounds__(256) void k_query2s(TreeView t, const float *querie
     for (int d = 0; d < D; d++) pa[d] = src_a[d];
 #pragma unroll
     for (int d = 0; d < D; d++) pb[d] = src_b[d];
-    sd_visit<D>(ra, pqa, pa, n, stride, sa, col_a, row_stride, visits);
-    sd_visit<D>(rb, pqb, pb, n, stride, sb, col_b, row_stride, visits);
+    sd_visit<D>(ra, pqa, pa, n_a, stride, sa, col_a, row_stride, visits);
+    sd_visit<D>(rb, pqb, pb, n_b, stride, sb, col_b, row_stride, visits);
   }
   ResultOps<R>::write(sta, qa, k, ids, dist2, ret, la);
   if (has_b) ResultOps<R>::write(stb, qb, k, ids, dist2, ret, la);
@@ -607,6 +618,7 @@ struct QueryArgs {
   int block = 128;  // workgroup size: 64, 128 or 256
   int lanes = 1;    // queries per lane: 1, or 2 (stack-based, register lists)
   uint64_t tree_bytes = 0;  // bytes of the tree array (forests: all trees)
+  int max_n = 0;            // forests: points of the largest tree (LDS rows of k_query2s)
   bool presorted = false;   // queries already in Morton order (CUKD_QUERY_PRESORTED)
 };
 
@@ -639,16 +651,25 @@ inline hipError_t launch_q(const QueryArgs &a, hipStream_t s)
                        a.nq >= QUERY2_FEW_WAVES_MIN_NQ;
       const size_t lds = few ? LDS_PER_CU / 7 : 0;
 #ifndef CUKD_NO_STORED_PLANES
-      // the same launch shape with the pending plane distances in the LDS
-      // that the cap reserves (a few more rows for deep trees)
-      if constexpr (FewWaves<R>::ok) {
-        if (few && !a.fq.seg_off) {
-          const size_t need = (size_t)sd_rows(a.t.n) * 2 * nt * sizeof(float);
-          hipLaunchKernelGGL((k_query2s<D, R>), dim3(blocks), dim3(nt), need > lds ? need : lds,
-                             s, a.t, a.queries, a.nq, a.cutoff, a.k, a.order, a.ids, a.dist2,
-                             a.ret, a.la, a.stats);
-          return hipGetLastError();
-        }
+      // Large batches, sorted or not, keep the pending plane distances in
+      // LDS.  The LDS rows for the deepest tree also bound the resident
+      // waves (6-7 workgroups per CU at 10M points), which suits unsorted
+      // batches too: fewer queries in flight thrash the L2s less.  A
+      // forest without a known largest tree keeps the kernel that re-reads
+      // parents.
+      if (nt == 128 && a.nq >= QUERY2_FEW_WAVES_MIN_NQ && (!a.fq.seg_off || a.max_n > 0)) {
+        const size_t need =
+            (size_t)sd_rows(a.fq.seg_off ? a.max_n : a.t.n) * 2 * nt * sizeof(float);
+        const size_t shm = need > lds ? need : lds;
+        if (a.fq.seg_off)
+          hipLaunchKernelGGL((k_query2s<D, R, true>), dim3(blocks), dim3(nt), shm, s, a.t,
+                             a.queries, a.nq, a.cutoff, a.k, a.order, a.ids, a.dist2, a.ret, a.la,
+                             a.fq, a.stats);
+        else
+          hipLaunchKernelGGL((k_query2s<D, R, false>), dim3(blocks), dim3(nt), shm, s, a.t,
+                             a.queries, a.nq, a.cutoff, a.k, a.order, a.ids, a.dist2, a.ret, a.la,
+                             a.fq, a.stats);
+        return hipGetLastError();
       }
 #endif
       if (a.fq.seg_off)

@@ -153,7 +153,7 @@ bool query_case(int n, int nq, int D, int method, int k, int flags, uint64_t see
   return ok;
 }
 
-bool forest_case(const std::vector<int64_t> &off, uint64_t seed)
+bool forest_case(const std::vector<int64_t> &off, uint64_t seed, int per = 40)
 {
   cukd_layout L{3, CUKD_SCALAR_F32, 12, 0, -1, 4};
   const int T = (int)off.size() - 1, N = (int)off.back();
@@ -169,8 +169,8 @@ bool forest_case(const std::vector<int64_t> &off, uint64_t seed)
   int rc = cukd_forest_create(&f, off.data(), T, &L, nullptr, nullptr);
   if (rc == CUKD_OK) rc = cukd_forest_build(f, d, nullptr, nullptr);
   bool ok = rc == CUKD_OK && !memcmp(d, ref.data(), e.size() * 4);
-  // kNN-4 over every tree, 40 queries each, against the oracle
-  const int per = 40, nq = per * T, k = 4;
+  // kNN-4 over every tree, per queries each, against the oracle
+  const int nq = per * T, k = 4;
   std::vector<float> q = queries(nq, 3, seed + 2);
   std::vector<int32_t> so(T + 1), st(T);
   for (int t = 0; t <= T; t++) so[t] = per * t;
@@ -304,6 +304,12 @@ int main(int argc, char **argv)
   cases.push_back({"query kNN-16 float4 sorted n=3000", [=] {
                      return query_case(3000, 1500, 4, CUKD_STACK_BASED, 16, CUKD_QUERY_SORT, 79);
                    }});
+  cases.push_back({"query FCP sorted n=6000", [=] {
+                     return query_case(6000, 3000, 3, CUKD_STACK_BASED, 0, CUKD_QUERY_SORT, 77);
+                   }});
+  cases.push_back({"query kNN-1 float4 sorted n=5000", [=] {
+                     return query_case(5000, 2100, 4, CUKD_STACK_BASED, 1, CUKD_QUERY_SORT, 78);
+                   }});
   for (int n : {1, 2, 3, 64})
     cases.push_back({"query kNN-8 sorted tiny n=" + std::to_string(n), [=] {
                        return query_case(n, 1100, 3, CUKD_STACK_BASED, 8, CUKD_QUERY_SORT, 83 + n);
@@ -316,6 +322,10 @@ int main(int argc, char **argv)
                    }});
   cases.push_back({"forest build + kNN-4", [=] {
                      return forest_case({0, 1, 700, 700, 5000, 5003, 12000}, 61);
+                   }});
+  // >= 1024 sorted queries: the large-batch kernel in the bigcap build
+  cases.push_back({"forest build + kNN-4 large sorted batch", [=] {
+                     return forest_case({0, 1, 700, 700, 5000, 5003, 12000}, 62, 200);
                    }});
   cases.push_back({"spatial float3 build + kNN-8", [=] { return spatial_case(6000, 3, CUKD_SCALAR_F32, 67); }});
   cases.push_back({"spatial int4 build", [=] { return spatial_case(6000, 4, CUKD_SCALAR_I32, 71); }});
