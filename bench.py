@@ -492,6 +492,8 @@ def main():
     ap.add_argument("--probe", action="store_true",
                     help="for hardware-counter runs: one 1 GiB device copy (the counters' "
                          "calibration), one build, one kNN and one FCP launch, no JSON line")
+    ap.add_argument("--probe-build", action="store_true",
+                    help="for kernel-trace runs: five builds of the 10M tree, no JSON line")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -520,6 +522,14 @@ def main():
     tree = torch.empty_like(pts)
     torch.cuda.synchronize()
     print("[bench] inputs resident: %d points, %d queries" % (n, nq), file=sys.stderr, flush=True)
+
+    if args.probe_build:
+        for _ in range(5):
+            tree.copy_(pts)
+            ck.buildTree(tree, lay)
+        torch.cuda.synchronize()
+        print("[bench] probe-build done", file=sys.stderr, flush=True)
+        return
 
     if args.probe:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB

@@ -262,12 +262,9 @@ __global__ __launch_bounds__(256) void k_copy_in(TreeSet<InTree> set, uint8_t *d
 // Per-level reset of the subtree states; round-robin dims and chains are
 // closed forms of the level.  One thread per segment of the level.
 __global__ void k_level_init(TreeSet<LvlTree> set, int nsegs, SegState *st,
-                             uint32_t *cursors, uint32_t *need, uint32_t *tb, int level, int D,
-                             int explicit_dim)
+                             uint32_t *cursors, uint32_t *tb, int level, int D, int explicit_dim)
 {
   const int sg = blockIdx.x * blockDim.x + threadIdx.x;
-  if (sg == 0)
-    for (int w = 0; w < MAX_WORDS; w++) need[w] = 0u;
   if (sg >= nsegs) return;
   const LvlTree tr = find_tree(set, sg, [](const LvlTree &x) { return x.hseg; });
   const int s = sg - tr.hseg;
@@ -317,9 +314,10 @@ __device__ inline int tiles_of(const LevelGeom &g, int s, int T)
 
 // The bin of h[0..nbins) that holds rank krem, into s_pick as (bin, count
 // of lower bins, count of the bin); every thread of the workgroup calls it.
-// Per-thread sums of consecutive bins, an inclusive scan over the threads,
-// then the owning thread walks its own few bins (a serial walk over all
-// bins would cost a dependent LDS load per bin).  nbins is a multiple of
+// The threads sum runs of nbins / SELECT_NT consecutive bins into s_part;
+// wave 0 alone then scans those sums (SELECT_NT / 64 per lane, then
+// shuffles across the lanes), and the owning lane walks down to the bin,
+// so the workgroup meets at two barriers only.  nbins is a multiple of
 // SELECT_NT.
 __device__ inline void pick_bin(const uint32_t *h, int nbins, uint32_t krem, uint32_t *s_part,
                                 uint32_t *s_pick)
@@ -330,56 +328,143 @@ __device__ inline void pick_bin(const uint32_t *h, int nbins, uint32_t krem, uin
   for (int i = 0; i < per; i++) mine += h[b0 + i];
   s_part[threadIdx.x] = mine;
   __syncthreads();
-  for (int off = 1; off < SELECT_NT; off <<= 1) {
-    const uint32_t add = (int)threadIdx.x >= off ? s_part[threadIdx.x - off] : 0u;
-    __syncthreads();
-    s_part[threadIdx.x] += add;
-    __syncthreads();
-  }
-  const uint32_t incl = s_part[threadIdx.x];
-  if (krem >= incl - mine && krem < incl) {
-    uint32_t before = incl - mine, cnt = 0;
-    int b = b0;
-    for (int i = 0; i < per; i++) {
-      cnt = h[b0 + i];
-      b = b0 + i;
-      if (before + cnt > krem) break;
-      before += cnt;
+  if (threadIdx.x < 64) {
+    enum { PL = SELECT_NT / 64 };
+    const int lane = (int)threadIdx.x;
+    uint32_t lsum = 0;
+    for (int i = 0; i < PL; i++) lsum += s_part[lane * PL + i];
+    uint32_t incl = lsum;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t v = __shfl_up(incl, off);
+      if (lane >= off) incl += v;
     }
-    s_pick[0] = (uint32_t)b;
-    s_pick[1] = before;
-    s_pick[2] = cnt;
+    if (krem >= incl - lsum && krem < incl) {
+      uint32_t before = incl - lsum;
+      int th = lane * PL;
+      for (int i = 0; i < PL; i++) {
+        th = lane * PL + i;
+        const uint32_t c = s_part[th];
+        if (before + c > krem) break;
+        before += c;
+      }
+      uint32_t cnt = 0;
+      int b = th * per;
+      for (int i = 0; i < per; i++) {
+        b = th * per + i;
+        cnt = h[b];
+        if (before + cnt > krem) break;
+        before += cnt;
+      }
+      s_pick[0] = (uint32_t)b;
+      s_pick[1] = before;
+      s_pick[2] = cnt;
+    }
   }
   __syncthreads();
 }
 
-// One digit pass of the radix select.  Every tile histograms the digit of
-// its candidates; the last tile of each subtree picks the digit bin that
-// holds the pivot rank.
-//
-// Cross-workgroup protocol: a tile adds its bins to the subtree's global
-// histogram with atomics; after the workgroup barrier its thread 0 bumps
-// the subtree's tile counter with an agent-scope release RMW.  The tile
-// that reads the final count issues an agent-scope acquire fence, so
-// (release sequence on the counter, fence synchronisation) every tile's
-// flush happens-before its read-back, which reads and clears the touched
-// bins with atomic exchanges.  Measured at 10M points: the release costs
-// 0.8 ms per build; the acquire, taken by one tile per subtree and pass,
-// costs about 0.1 ms.
-__global__ __launch_bounds__(SELECT_NT) void k_select_pass(
-    TreeSet<LvlTree> set, int level, int word, int shift, int bits,
-    const uint32_t *key, const uint8_t *data, const uint32_t *idx,
-    SegState *st, uint32_t *hist, uint32_t *need, cukd_layout L, int explicit_dim, int hcopies)
+// Child domains and split dims of an explicit-dim subtree once its pivot's
+// split coordinate (key word 0) is known (updateTagAndSetDim,
+// builder_thrust.h:259-314): the pivot coordinate clips the parent box on
+// the split dim, and each child splits its widest dim.
+__device__ inline void write_child_domains(const SegState &t, SegState *child_st, int s,
+                                           const cukd_layout &L, bool isf)
 {
-  if (word > 0 && __hip_atomic_load(&need[word], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
-    return;
+  const int D = L.num_dims;
+  const uint32_t pc = unordered_key(t.fixed[0], isf);
+  for (int side = 0; side < 2; side++) {
+    SegState &c = child_st[2 * s + side];
+    uint32_t box[2 * CUKD_MAX_DIMS];
+    for (int i = 0; i < 2 * D; i++) box[i] = t.box[i];
+    box[(side == 0 ? D : 0) + (int)t.dim] = pc;
+    const int cd = widest_dim_bits(box, D, isf);
+    for (int i = 0; i < 2 * D; i++) c.box[i] = box[i];
+    c.dim = (uint32_t)cd;
+    child_chain(t.chain, t.chain_len, (uint32_t)cd, c.chain, c.chain_len);
+  }
+}
+
+// Digit histograms of key word 0: digit d (bits 31-21, 20-10, 9-0) of
+// subtree s, copy c, is row (hseg + s) * hcopies + c of buffer d: the bins,
+// then one bitmap bit per bin that some tile touched.
+__device__ inline const uint32_t *hist_rows(const uint32_t *h, int row0, int hcopies)
+{
+  return h + (size_t)row0 * hcopies * HIST_STRIDE;
+}
+
+// The subtree's digit histogram summed over its copies into h[0..nbins);
+// only bins that some tile touched are read.  Every thread calls it.
+__device__ inline void load_hist(uint32_t *h, const uint32_t *rows, int hcopies, int nbins)
+{
+  const int per = nbins / SELECT_NT;  // 2 or 4 consecutive bins, within one bitmap word
+  const int b0 = (int)threadIdx.x * per;
+  uint32_t sum[DIGIT_MAX_BINS / SELECT_NT];
+  for (int i = 0; i < per; i++) sum[i] = 0u;
+  for (int c = 0; c < hcopies; c++) {
+    const uint32_t *r = rows + (size_t)c * HIST_STRIDE;
+    const uint32_t bits = (r[DIGIT_MAX_BINS + (b0 >> 5)] >> (b0 & 31)) & ((1u << per) - 1u);
+    for (int i = 0; i < per; i++)
+      if ((bits >> i) & 1u) sum[i] += r[b0 + i];
+  }
+  for (int i = 0; i < per; i++) h[b0 + i] = sum[i];
+}
+
+// Word 0's digit pick d of one subtree, replayed from the histogram that
+// digit pass d left (its atomics were performed before this launch
+// began): krem and prefix advance past digit d.  Returns the count of the
+// picked bin.  Every thread calls it and gets the same result; s_h is LDS
+// scratch of DIGIT_MAX_BINS words.
+__device__ inline uint32_t replay_digit(uint32_t &krem, uint32_t &prefix, const uint32_t *hd,
+                                        int row0, int hcopies, int d, uint32_t *s_h,
+                                        uint32_t *s_part, uint32_t *s_pick)
+{
+  const int bits = d == 2 ? 10 : 11;
+  const int nbins = 1 << bits;
+  load_hist(s_h, hist_rows(hd, row0, hcopies), hcopies, nbins);
+  __syncthreads();
+  // pick_bin reads s_h and ends with a barrier, so s_h may be reused after
+  pick_bin(s_h, nbins, krem, s_part, s_pick);
+  krem -= s_pick[1];
+  prefix = (prefix << bits) | s_pick[0];
+  return s_pick[2];
+}
+
+// The state of word 0's select entering digit d of subtree s: for d = 0
+// the level's start; after that, what the launch of digit pass d recorded
+// in rk when it replayed digit d - 1 (per subtree of the level: krem and
+// prefix after digit 0, then after digit 1).  It is written by the tile
+// holding the subtree's first element and read only by later launches.
+__device__ inline void word0_state(int d, const SegState *st, const uint32_t *rk, int s,
+                                   int row, uint32_t &krem, uint32_t &prefix)
+{
+  if (d == 0) {
+    krem = st[s].krem;
+    prefix = 0u;
+  } else {
+    krem = rk[(size_t)row * 4 + 2 * (d - 1)];
+    prefix = rk[(size_t)row * 4 + 2 * (d - 1) + 1];
+  }
+}
+
+// Digit pass d of the radix select on key word 0 (the split coordinate),
+// over every subtree of the level.  Every tile first replays the picks of
+// the earlier digits from their histograms, then counts digit d of its
+// candidates (the elements whose key agrees with those picks) and adds its
+// non-empty bins, plus their bitmap words, to the subtree's digit-d
+// histogram with atomics.  No tile reads another tile's counts in the same
+// launch: the next launch picks, after the kernel boundary has made every
+// tile's atomics visible, so the passes need no fences.
+//
+// Each 16K-key tile loads 8 keys per thread before using any; a wave whose
+// counted lanes all hit one bin does a single LDS atomic.  Subtrees with
+// many tiles spread their atomics over hcopies histogram copies.
+__global__ __launch_bounds__(SELECT_NT) void k_select_digit(
+    TreeSet<LvlTree> set, int level, int d, const uint32_t *key, const SegState *st,
+    uint32_t *h0, uint32_t *h1, uint32_t *h2, uint32_t *rk, int hcopies)
+{
   __shared__ uint32_t s_hist[2][DIGIT_MAX_BINS];
-  __shared__ SegState s_st[2];
-  __shared__ int s_seg[2];
-  __shared__ int s_last[2];
   __shared__ uint32_t s_part[SELECT_NT];
   __shared__ uint32_t s_pick[3];
-  __shared__ uint32_t s_bm[SEL_HCOPIES][DIGIT_MAX_BINS / 32];
   __shared__ LvlTree s_tree;
   if (threadIdx.x == 0)
     s_tree = find_tree(set, (int)blockIdx.x, [](const LvlTree &x) { return x.stile0; });
@@ -389,34 +474,40 @@ __global__ __launch_bounds__(SELECT_NT) void k_select_pass(
   g.init(tr.n, level);
   const int T = tr.tsel;
   key += tr.base;
-  data += (size_t)tr.base * L.stride;
-  idx += tr.base;
-  SegState *child_st = st + tr.cseg;
   st += tr.seg;
-  // each subtree owns hcopies histograms; a tile flushes into one of them,
-  // so few tiles add into the same counters (levels with few subtrees)
-  hist += (size_t)tr.hseg * hcopies * HIST_STRIDE;
-  const int hcopy = (int)blockIdx.x & (hcopies - 1);
-  const bool isf = L.scalar_type == CUKD_SCALAR_F32;
-  const int nbins = 1 << bits;
   const int t0 = ((int)blockIdx.x - tr.stile0) * T;
   const int t1 = min(t0 + T, g.n_elems);
   const int seg0 = g.seg_of(t0);
   const int seg1 = g.seg_of(t1 - 1);
-  for (int i = threadIdx.x; i < 2 * nbins; i += blockDim.x) s_hist[i / nbins][i % nbins] = 0u;
-  if (threadIdx.x == 0) { s_seg[0] = seg0; s_seg[1] = seg1; s_st[0] = st[seg0]; }
-  if (threadIdx.x == 1) s_st[1] = st[seg1];
-  __syncthreads();
-  // tiles inside one subtree use slot 0 only
-  const int seg1_begin = seg1 != seg0 ? g.offset(seg1) : g.n_elems;
+  const int nslots = seg1 != seg0 ? 2 : 1;
+  // the earlier digits' picks: the last one replayed here, and recorded
+  // for the next launch by the tile holding the subtree's first element
+  uint32_t prefix[2] = {0u, 0u};
+  if (d > 0) {
+    for (int sl = 0; sl < nslots; sl++) {
+      const int s = sl ? seg1 : seg0;
+      uint32_t krem;
+      word0_state(d - 1, st, rk, s, tr.hseg + s, krem, prefix[sl]);
+      replay_digit(krem, prefix[sl], d == 1 ? h0 : h1, tr.hseg + s, hcopies, d - 1,
+                   &s_hist[0][0], s_part, s_pick);
+      if (threadIdx.x == 0 && (sl == 1 || g.offset(s) >= t0)) {
+        rk[(size_t)(tr.hseg + s) * 4 + 2 * (d - 1)] = krem;
+        rk[(size_t)(tr.hseg + s) * 4 + 2 * (d - 1) + 1] = prefix[sl];
+      }
+    }
+  }
+  const int shift = d == 0 ? 21 : (d == 1 ? 10 : 0);
+  const int bits = d == 2 ? 10 : 11;
+  const int nbins = 1 << bits;
   const uint32_t mask = (uint32_t)nbins - 1u;
   const int hi_shift = shift + bits;
-  // tiles whose subtrees are resolved or on another word only flush zeros
-  const bool act0 = !s_st[0].resolved && (int)s_st[0].word == word;
-  const bool act1 = seg1 != seg0 && !s_st[1].resolved && (int)s_st[1].word == word;
+  for (int i = threadIdx.x; i < 2 * nbins; i += blockDim.x) s_hist[i / nbins][i % nbins] = 0u;
+  __syncthreads();
+  // tiles inside one subtree use slot 0 only
+  const int seg1_begin = nslots == 2 ? g.offset(seg1) : g.n_elems;
   const int lane = (int)threadIdx.x & 63;
   uint32_t *hflat = &s_hist[0][0];
-  for (int base = t0; (act0 || act1) && base < t1; base += SELECT_NT * SEL_BATCH) {
+  for (int base = t0; base < t1; base += SELECT_NT * SEL_BATCH) {
     // all of a thread's key loads in flight before any is used
     uint32_t kv[SEL_BATCH];
 #pragma unroll
@@ -428,21 +519,8 @@ __global__ __launch_bounds__(SELECT_NT) void k_select_pass(
     for (int u = 0; u < SEL_BATCH; u++) {
       const int p = base + u * SELECT_NT + (int)threadIdx.x;
       const int slot = p >= seg1_begin ? 1 : 0;
-      const SegState &t = s_st[slot];
-      bool has = p < t1 && (slot ? act1 : act0);
-      uint32_t v = kv[u];
-      if (has && word > 0) {
-        // candidates agree with the pivot on every earlier word
-        has = kv[u] == t.fixed[0];
-        if (has) {
-          const uint8_t *elem = data + (size_t)p * L.stride;
-          const uint32_t id = idx[p];
-          for (int w = 1; w < word && has; w++)
-            has = key_word(w, t, kv[u], elem, id, L, isf) == t.fixed[w];
-          if (has) v = key_word(word, t, kv[u], elem, id, L, isf);
-        }
-      }
-      if (has && hi_shift < 32 && (v >> hi_shift) != t.prefix) has = false;
+      const uint32_t v = kv[u];
+      const bool has = p < t1 && (hi_shift >= 32 || (v >> hi_shift) == prefix[slot]);
       const uint32_t hb = (uint32_t)slot * DIGIT_MAX_BINS + ((v >> shift) & mask);
       // one LDS atomic per wave when every counted lane hits the same bin
       // (the common case deep in the tree, where a subtree's keys share
@@ -461,10 +539,11 @@ __global__ __launch_bounds__(SELECT_NT) void k_select_pass(
     }
   }
   __syncthreads();
-  const int nslots = seg1 != seg0 ? 2 : 1;
+  uint32_t *hd = d == 0 ? h0 : (d == 1 ? h1 : h2);
+  const int hcopy = (int)blockIdx.x & (hcopies - 1);
   for (int sl = 0; sl < nslots; sl++) {
-    const int s = s_seg[sl];
-    uint32_t *h = hist + ((size_t)s * hcopies + hcopy) * HIST_STRIDE;
+    const int s = sl ? seg1 : seg0;
+    uint32_t *h = hd + ((size_t)(tr.hseg + s) * hcopies + hcopy) * HIST_STRIDE;
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
       const uint32_t c = s_hist[sl][b];
       if (c) atomicAdd(&h[b], c);
@@ -474,91 +553,15 @@ __global__ __launch_bounds__(SELECT_NT) void k_select_pass(
       if ((lane & 31) == 0 && part) atomicOr(&h[DIGIT_MAX_BINS + (b >> 5)], part);
     }
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int sl = 0; sl < 2; sl++) {
-      s_last[sl] = 0;
-      if (sl < nslots) {
-        const int s = s_seg[sl];
-        // release at agent scope: every thread's flush (ordered before
-        // this by the barrier) happens-before the RMW; the tile that reads
-        // the final count acquires, so all tiles' flushes happen-before
-        // its read-back (release sequence on tiles_done)
-        const uint32_t done = __hip_atomic_fetch_add(&st[s].tiles_done, 1u, __ATOMIC_RELEASE,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-        s_last[sl] = (int)done == tiles_of(g, s, T) - 1;
-        if (s_last[sl]) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      }
-    }
-  }
-  __syncthreads();
-  for (int sl = 0; sl < nslots; sl++) {
-    if (!s_last[sl]) continue;
-    const int s = s_seg[sl];
-    uint32_t *h = hist + (size_t)s * hcopies * HIST_STRIDE;
-    // reuse s_hist[sl] for the subtree's global histogram.  Read-modify-
-    // write atomics are performed where every tile's flush was (plain or
-    // relaxed loads may be served by a stale copy in this XCD's L2), and
-    // the bitmaps limit them to the bins some tile touched; reading with
-    // exchanges also clears the counters for the next pass.
-    for (int i = threadIdx.x; i < hcopies * (nbins / 32); i += blockDim.x) {
-      const int c = i / (nbins / 32), w = i % (nbins / 32);
-      s_bm[c][w] = atomicExch(&h[(size_t)c * HIST_STRIDE + DIGIT_MAX_BINS + w], 0u);
-    }
-    __syncthreads();
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
-      uint32_t sum = 0;
-      for (int c = 0; c < hcopies; c++)
-        if ((s_bm[c][b >> 5] >> (b & 31)) & 1u) sum += atomicExch(&h[(size_t)c * HIST_STRIDE + b], 0u);
-      s_hist[sl][b] = sum;
-    }
-    __syncthreads();
-    if (!s_st[sl].resolved && (int)s_st[sl].word == word)
-      pick_bin(s_hist[sl], nbins, s_st[sl].krem, s_part, s_pick);
-    if (threadIdx.x == 0) {
-      SegState t = s_st[sl];
-      t.tiles_done = 0;
-      if (!t.resolved && (int)t.word == word) {
-        const uint32_t b = s_pick[0], before = s_pick[1], cnt = s_pick[2];
-        t.krem -= before;
-        t.prefix = (hi_shift < 32 ? (t.prefix << bits) : 0u) | b;
-        if (shift == 0) {
-          t.fixed[word] = t.prefix;
-          t.prefix = 0;
-          if (cnt == 1u) {
-            t.resolved = 1;
-          } else if (word + 1 < MAX_WORDS) {
-            t.word = (uint32_t)(word + 1);
-            __hip_atomic_store(&need[word + 1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          }
-          if (word == 0 && explicit_dim) {
-            // child domains and dims (updateTagAndSetDim,
-            // builder_thrust.h:259-314): the pivot coordinate clips the
-            // parent box on the split dim.
-            const int D = L.num_dims;
-            const uint32_t pc = unordered_key(t.fixed[0], isf);
-            for (int side = 0; side < 2; side++) {
-              SegState &c = child_st[2 * s + side];
-              uint32_t box[2 * CUKD_MAX_DIMS];
-              for (int i = 0; i < 2 * D; i++) box[i] = t.box[i];
-              box[(side == 0 ? D : 0) + (int)t.dim] = pc;
-              const int cd = widest_dim_bits(box, D, isf);
-              for (int i = 0; i < 2 * D; i++) c.box[i] = box[i];
-              c.dim = (uint32_t)cd;
-              child_chain(t.chain, t.chain_len, (uint32_t)cd, c.chain, c.chain_len);
-            }
-          }
-        }
-      }
-      st[s] = t;
-    }
-    __syncthreads();
-  }
 }
 
-// Tied pivots of a level, in one launch.  After the word-0 passes, a
-// subtree that is not resolved holds several elements whose key word 0
-// equals its pivot's.  Its pivot is the one of rank krem among them in the
+// Pivots of a level: word 0's last pick, then the tied subtrees, in one
+// launch.  Every tile replays word 0's three digit picks for its subtrees.
+// A subtree whose pivot key is unique is done: the tile that holds its
+// first element writes its final state (to st_fin, which no tile of this
+// launch reads) and, for explicit dims, its children's domains.  A subtree
+// that is not resolved holds several elements whose key word 0 equals its
+// pivot's.  Its pivot is the one of rank krem among them in the
 // order of the remaining key words: the split-dim chain's earlier
 // coordinates, then the input index, which is the order the host builder's
 // stable sorts leave (builder_thrust.h:126-142).
@@ -569,8 +572,12 @@ __global__ __launch_bounds__(SELECT_NT) void k_select_pass(
 // the subtree's own range of a buffer that the level leaves unused (the
 // next level's element array, which has at least as many words per element
 // as there are tie words), so a group of any size fits.  The tile that
-// completes the subtree (the counter protocol of k_select_pass) then runs
-// the digit passes over those planes alone, inside its workgroup.  The
+// completes the subtree then runs the digit passes over those planes
+// alone, inside its workgroup.  Protocol: after the workgroup barrier,
+// thread 0 of every tile bumps the subtree's tile counter with an
+// agent-scope release RMW; the tile that reads the final count issues an
+// agent-scope acquire fence, so (release sequence on the counter, fence
+// synchronisation) every tile's appends happen-before its reads.  The
 // gather also keeps each tie word's smallest and largest value over the
 // group (tb, per subtree of the level): a word on which the whole group
 // agrees is fixed without a pass, and so is every leading digit that its
@@ -579,10 +586,9 @@ __global__ __launch_bounds__(SELECT_NT) void k_select_pass(
 enum { TIE_BATCH = 8 };
 __global__ __launch_bounds__(SELECT_NT) void k_tie_resolve(
     TreeSet<LvlTree> set, int level, const uint32_t *key, const uint8_t *data,
-    const uint32_t *idx, SegState *st, uint8_t *scratch, const uint32_t *need, uint32_t *tb,
-    cukd_layout L)
+    const uint32_t *idx, SegState *st, SegState *st_fin, const uint32_t *h2, const uint32_t *rk,
+    int hcopies, uint8_t *scratch, uint32_t *tb, cukd_layout L, int explicit_dim)
 {
-  if (__hip_atomic_load(&need[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
   enum { NW = SELECT_NT / 64 };
   __shared__ uint32_t s_hist[DIGIT_MAX_BINS];
   __shared__ SegState s_st[2];
@@ -604,7 +610,9 @@ __global__ __launch_bounds__(SELECT_NT) void k_tie_resolve(
   data += (size_t)tr.base * L.stride;
   idx += tr.base;
   uint32_t *words = (uint32_t *)(scratch + (size_t)tr.base * L.stride);
+  SegState *child_st = st + tr.cseg;
   st += tr.seg;
+  st_fin += tr.hseg;
   const bool isf = L.scalar_type == CUKD_SCALAR_F32;
   const int t0 = ((int)blockIdx.x - tr.stile0) * T;
   const int t1 = min(t0 + T, g.n_elems);
@@ -619,6 +627,34 @@ __global__ __launch_bounds__(SELECT_NT) void k_tie_resolve(
   __syncthreads();
   // tiles inside one subtree use slot 0 only
   const int nslots = seg1 != seg0 ? 2 : 1;
+  // word 0's picks: the pivot's key word 0, and its rank among the
+  // elements that share it
+  for (int sl = 0; sl < nslots; sl++) {
+    const int s = sl ? seg1 : seg0;
+    uint32_t krem, key0;
+    word0_state(2, st, rk, s, tr.hseg + s, krem, key0);
+    const uint32_t cnt =
+        replay_digit(krem, key0, h2, tr.hseg + s, hcopies, 2, s_hist, s_part, s_pick);
+    if (threadIdx.x == 0) {
+      SegState &t = s_st[sl];
+      t.krem = krem;
+      t.fixed[0] = key0;
+      t.prefix = 0u;
+      t.resolved = cnt == 1u ? 1u : 0u;
+      t.word = cnt == 1u ? 0u : 1u;
+    }
+    __syncthreads();
+  }
+  // the tile holding a subtree's first element writes what depends on key
+  // word 0 alone
+  if (threadIdx.x < 2 && (int)threadIdx.x < nslots) {
+    const int sl = (int)threadIdx.x;
+    const int s = sl ? seg1 : seg0;
+    if (g.offset(s) >= t0) {
+      if (explicit_dim) write_child_domains(s_st[sl], child_st, s, L, isf);
+      if (s_st[sl].resolved) st_fin[s] = s_st[sl];
+    }
+  }
   const bool tied0 = !s_st[0].resolved;
   const bool tied1 = nslots == 2 && !s_st[1].resolved;
   if (!tied0 && !tied1) return;
@@ -808,7 +844,7 @@ __global__ __launch_bounds__(SELECT_NT) void k_tie_resolve(
       t.tiles_done = 0;
       t.gcount = 0;
       t.prefix = 0;
-      st[s] = t;
+      st_fin[s] = t;
     }
     __syncthreads();
   }
@@ -828,8 +864,9 @@ __global__ __launch_bounds__(SELECT_NT) void k_tie_resolve(
 template <int SW>
 __global__ __launch_bounds__(SEL_NT) void k_partition(
     TreeSet<LvlTree> set, int level, const uint32_t *key, const uint8_t *data,
-    const uint32_t *idx, const SegState *st, uint8_t *ndata, uint32_t *nidx, uint32_t *nkey,
-    uint32_t *cursors, uint8_t *out, cukd_layout L, int explicit_dim)
+    const uint32_t *idx, const SegState *st, const SegState *st_fin, uint8_t *ndata,
+    uint32_t *nidx, uint32_t *nkey, uint32_t *cursors, uint8_t *out, cukd_layout L,
+    int explicit_dim, uint32_t *h0, uint32_t *h1, uint32_t *h2, int hrows)
 {
   enum { TILE = SEL_ITEMS * SEL_NT, STAGED = SW > 0 };
   __shared__ SegState s_st[2];
@@ -865,15 +902,32 @@ __global__ __launch_bounds__(SEL_NT) void k_partition(
   out += (size_t)tr.base * L.stride;
   cursors += 2 * tr.hseg;
   const SegState *child_st = st + tr.cseg;
-  st += tr.seg;
+  st_fin += tr.hseg;
   const bool isf = L.scalar_type == CUKD_SCALAR_F32;
   const int D = L.num_dims;
   const int t0 = ((int)blockIdx.x - tr.ptile0) * T;
   const int t1 = min(t0 + T, g.n_elems);
   const int seg0 = g.seg_of(t0);
   const int seg1 = g.seg_of(t1 - 1);
-  if (threadIdx.x == 0) s_st[0] = st[seg0];
-  if (threadIdx.x == 1) s_st[1] = st[seg1];
+  if (threadIdx.x == 0) s_st[0] = st_fin[seg0];
+  if (threadIdx.x == 1) s_st[1] = st_fin[seg1];
+  // this level's digit histograms are read for the last time by now: clear
+  // the touched bins (bitmaps) for the next level, a few rows per tile
+  for (int r = (int)blockIdx.x; r < hrows; r += (int)gridDim.x) {
+    if (threadIdx.x < 3 * (DIGIT_MAX_BINS / 32)) {
+      const int hsel = (int)threadIdx.x / (DIGIT_MAX_BINS / 32);
+      const int w = (int)threadIdx.x % (DIGIT_MAX_BINS / 32);
+      uint32_t *row = (hsel == 0 ? h0 : (hsel == 1 ? h1 : h2)) + (size_t)r * HIST_STRIDE;
+      uint32_t bm = row[DIGIT_MAX_BINS + w];
+      if (bm) {
+        row[DIGIT_MAX_BINS + w] = 0u;
+        while (bm) {
+          row[w * 32 + __builtin_ctz(bm)] = 0u;
+          bm &= bm - 1u;
+        }
+      }
+    }
+  }
   if (threadIdx.x < 4) {
     const int sl = threadIdx.x >> 1, side = threadIdx.x & 1;
     const int s = sl ? seg1 : seg0;
@@ -1807,9 +1861,10 @@ int make_forest_plan(ForestPlan &P, const long long *off, int ntrees, const cukd
     P.off_dataB = P.off_idxB = P.off_keyB = 0;
   }
   P.off_st = o; o = align_up(o + P.nstates * sizeof(SegState));
-  P.off_hist = o; o = align_up(o + P.max_hseg * HIST_STRIDE * 4);
+  P.off_hist = o; o = align_up(o + 3 * P.max_hseg * HIST_STRIDE * 4);  // digits 0, 1, 2
+  P.off_stfin = o; o = align_up(o + P.max_hseg * sizeof(SegState));
+  P.off_rk = o; o = align_up(o + P.max_hseg * 4 * 4);
   P.off_cursors = o; o = align_up(o + P.max_hseg * 2 * 4);
-  P.off_need = o; o = align_up(o + MAX_WORDS * 4);
   P.off_tb = o; o = align_up(o + P.max_hseg * 2 * MAX_WORDS * 4);
   P.zero_end = o;
   P.off_acc = o; o = align_up(o + (size_t)ntrees * 2 * D * 4);
@@ -1869,9 +1924,12 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
   uint32_t *idxA = (uint32_t *)(ws + P.off_idxA), *idxB = (uint32_t *)(ws + P.off_idxB);
   uint32_t *keyA = (uint32_t *)(ws + P.off_keyA), *keyB = (uint32_t *)(ws + P.off_keyB);
   SegState *st = (SegState *)(ws + P.off_st);
-  uint32_t *hist = (uint32_t *)(ws + P.off_hist);
+  uint32_t *h0 = (uint32_t *)(ws + P.off_hist);
+  uint32_t *h1 = h0 + P.max_hseg * HIST_STRIDE;
+  uint32_t *h2 = h1 + P.max_hseg * HIST_STRIDE;
+  SegState *stfin = (SegState *)(ws + P.off_stfin);
+  uint32_t *rk = (uint32_t *)(ws + P.off_rk);
   uint32_t *cursors = (uint32_t *)(ws + P.off_cursors);
-  uint32_t *need = (uint32_t *)(ws + P.off_need);
   uint32_t *tb = (uint32_t *)(ws + P.off_tb);
   uint32_t *acc = (uint32_t *)(ws + P.off_acc);
   uint8_t *pts = (uint8_t *)d_points;
@@ -1899,36 +1957,34 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
     const TreeSet<LvlTree> set =
         make_set(P, ws, P.tables ? P.off_tab_lvl[level] : 0, P.lvl[level]);
     hipLaunchKernelGGL(k_level_init, dim3((P.lvl_segs[level] + 255) / 256), dim3(256), 0, s, set,
-                       P.lvl_segs[level], st, cursors, need, tb, level, D, expl ? 1 : 0);
-    const int max_words = expl ? D + 1 : (level + 1 < D ? level + 1 : D) + 1;
+                       P.lvl_segs[level], st, cursors, tb, level, D, expl ? 1 : 0);
     // histogram copies per subtree where many tiles add into one subtree
-    // (at least 64 tiles per copy: the last tile reads every copy, and 32 or
-    // 16 tiles per copy measured slower at 10M points), as many as the
+    // (at least 64 tiles per copy: the later passes read every copy, and 32
+    // or 16 tiles per copy measured slower at 10M points), as many as the
     // buffer (sized for the level with the most subtrees) holds
     const int tiles_per_seg = P.lvl_stiles[level] / (P.lvl_segs[level] > 0 ? P.lvl_segs[level] : 1);
     int hcopies = 1;
     while (hcopies < SEL_HCOPIES && tiles_per_seg >= 64 * hcopies &&
            (size_t)P.lvl_segs[level] * hcopies * 2 <= P.max_hseg)
       hcopies *= 2;
-    // key word 0 (the split coordinate) over the whole level, then the
-    // few tied subtrees over their tie groups alone, in one launch
-    const int shifts[3] = {21, 10, 0};
-    const int bitsv[3] = {11, 11, 10};
+    // key word 0 (the split coordinate) over the whole level, one digit
+    // per launch; then the last pick and the few tied subtrees (over their
+    // tie groups alone) in one launch
     for (int d = 0; d < 3; d++)
-      hipLaunchKernelGGL(k_select_pass, dim3(P.lvl_stiles[level]), dim3(SELECT_NT), 0, s, set,
-                         level, 0, shifts[d], bitsv[d], (const uint32_t *)kcur,
-                         (const uint8_t *)dcur, (const uint32_t *)icur, st, hist, need, L,
-                         expl ? 1 : 0, hcopies);
-    if (max_words > 1)
-      hipLaunchKernelGGL(k_tie_resolve, dim3(P.lvl_stiles[level]), dim3(SELECT_NT), 0, s, set,
-                         level, (const uint32_t *)kcur, (const uint8_t *)dcur,
-                         (const uint32_t *)icur, st, dnext, (const uint32_t *)need, tb, L);
+      hipLaunchKernelGGL(k_select_digit, dim3(P.lvl_stiles[level]), dim3(SELECT_NT), 0, s, set,
+                         level, d, (const uint32_t *)kcur, (const SegState *)st, h0, h1, h2, rk,
+                         hcopies);
+    hipLaunchKernelGGL(k_tie_resolve, dim3(P.lvl_stiles[level]), dim3(SELECT_NT), 0, s, set,
+                       level, (const uint32_t *)kcur, (const uint8_t *)dcur,
+                       (const uint32_t *)icur, st, stfin, (const uint32_t *)h2,
+                       (const uint32_t *)rk, hcopies, dnext, tb, L, expl ? 1 : 0);
     const int W = L.stride >> 2;
     auto kp = W <= 3 ? k_partition<3> : (W == 4 ? k_partition<4> : k_partition<0>);
     hipLaunchKernelGGL(kp, dim3(P.lvl_ptiles[level]), dim3(SEL_NT), 0, s, set, level,
                        (const uint32_t *)kcur, (const uint8_t *)dcur, (const uint32_t *)icur,
-                       (const SegState *)st, dnext, inext, knext, cursors, pts, L,
-                       expl ? 1 : 0);
+                       (const SegState *)st, (const SegState *)stfin, dnext, inext, knext,
+                       cursors, pts, L, expl ? 1 : 0, h0, h1, h2,
+                       P.lvl_segs[level] * hcopies);
     uint8_t *t8 = dcur; dcur = dnext; dnext = t8;
     uint32_t *t32 = icur; icur = inext; inext = t32;
     t32 = kcur; kcur = knext; knext = t32;
