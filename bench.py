@@ -785,8 +785,9 @@ def main():
                      "unit": "GB/s", "frac": round(knn_gbs / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "traffic_source": traffic_note,
-                     "kernel": "k_query2<3, FixedCandidateList<8>> (stack-based, two queries "
-                               "per lane) on Morton-ordered queries",
+                     "kernel": "k_query2s<3, FixedCandidateList<8>> (stack-based, two queries "
+                               "per lane, pending plane distances in LDS) on Morton-ordered "
+                               "queries",
                      "algorithmic_bytes_per_launch": knn_bytes,
                      "avg_launch_ms": round(kernel_ms, 4),
                      "step_ms_with_ordering": round(ev_ms, 4)},

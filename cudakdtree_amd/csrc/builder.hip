@@ -1320,6 +1320,7 @@ struct RankLDS {
   uint32_t kbuf[CAP];              // radix key of each slot
   uint32_t cnt[RK_CSTRIDE * (NT / 64)];  // radix: counts at [wave*RK_CSTRIDE + digit]; partition: subtree prefixes
   uint32_t wsum[NT / 64];
+  uint32_t rlo[NT / 64], rhi[NT / 64];  // per-wave key bounds of rk_load_keys
   uint16_t pivot[CAP];             // slot of each subtree node, by local BFS index
   uint8_t side[CAP];               // per slot: 0 left, 1 right, 2 pivot
 };
@@ -1409,32 +1410,58 @@ __device__ inline void rk_radix_pass(RankLDS<CAP, NT, DM> &S, const uint16_t *sr
   __syncthreads();
 }
 
-// kbuf[slot] = the slot's input index (d < 0) or ordered coordinate d
+// kbuf[slot] = the slot's input index (d < 0) or ordered coordinate d.
+// Returns how many low key bits vary over the m keys: an LSD sort needs
+// no pass over the digits above them (every key shares those).
 template <int CAP, int NT, int DM>
-__device__ inline void rk_load_keys(RankLDS<CAP, NT, DM> &S, const uint8_t *sdata, const uint32_t *sidx,
-                                    int m, int d, const cukd_layout &L, bool isf)
+__device__ inline int rk_load_keys(RankLDS<CAP, NT, DM> &S, const uint8_t *sdata, const uint32_t *sidx,
+                                   int m, int d, const cukd_layout &L, bool isf)
 {
-  enum { E = CAP / NT };
+  enum { E = CAP / NT, NW = NT / 64 };
   uint32_t v[E];
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int i = e * NT + threadIdx.x;
     v[e] = i < m ? (d < 0 ? sidx[i] : load_coord_bits(sdata + (size_t)i * L.stride, L, d)) : 0u;
   }
+  // the previous keys and bounds may still be read (a sort of no passes
+  // has no barrier of its own)
+  __syncthreads();
+  uint32_t lo = 0xffffffffu, hi = 0u;
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int i = e * NT + threadIdx.x;
-    if (i < m) S.kbuf[i] = d < 0 ? v[e] : ordered_key(v[e], isf);
+    if (i < m) {
+      const uint32_t k = d < 0 ? v[e] : ordered_key(v[e], isf);
+      S.kbuf[i] = k;
+      lo = min(lo, k);
+      hi = max(hi, k);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, (uint32_t)__shfl_xor((int)lo, o));
+    hi = max(hi, (uint32_t)__shfl_xor((int)hi, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    S.rlo[threadIdx.x >> 6] = lo;
+    S.rhi[threadIdx.x >> 6] = hi;
   }
   __syncthreads();
+  for (int w = 0; w < NW; w++) {
+    lo = min(lo, S.rlo[w]);
+    hi = max(hi, S.rhi[w]);
+  }
+  return lo >= hi ? 0 : 32 - __clz(lo ^ hi);
 }
 
 // the buffer holding order d (4 bits per order in 'map')
 __device__ inline int rk_buf(uint32_t map, int d) { return (int)((map >> (4 * d)) & 15u); }
 
 // Stable LSD sort of buffer 'a' by the kbuf key, using buffer 't' as
-// scratch, skipping digit passes at or above 'bits'.  Returns the buffer
-// that holds the result (a or t) and leaves the other as scratch.
+// scratch, skipping digit passes at or above 'bits' (digits every key
+// shares leave a stable order unchanged).  Leaves the result in a and the
+// other buffer as scratch in t.
 template <int CAP, int NT, int DM>
 __device__ inline void rk_sort(RankLDS<CAP, NT, DM> &S, int &a, int &t, int m, int bits)
 {
@@ -1474,8 +1501,6 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
   const int base = L0 > 0 ? g.offset(s) : 0;
   const uint8_t *sdata = data + (size_t)base * L.stride;
   const uint32_t *sidx = idx + base;
-  // input indices are < n: higher idx digits are zero
-  const int idx_bits = n > 1 ? 32 - __clz(n - 1) : 1;
 
   // Fast path: order each dim by its coordinate alone (four passes), then
   // reorder every run of equal coordinates by the rest of its key.  Runs
@@ -1487,8 +1512,7 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
   for (int d = 0; d < D; d++) {
     int a = d;
     for (int i = threadIdx.x; i < m; i += NT) S.buf[a][i] = (uint16_t)i;
-    rk_load_keys(S, sdata, sidx, m, d, L, isf);
-    rk_sort(S, a, t, m, 32);  // four passes: the result is back in buffer d
+    rk_sort(S, a, t, m, rk_load_keys(S, sdata, sidx, m, d, L, isf));
     uint16_t *A = S.buf[a];
 #pragma unroll
     for (int e = 0; e < E; e++) {
@@ -1536,12 +1560,8 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
     int a = 0;
     t = DM;
     for (int i = threadIdx.x; i < m; i += NT) S.buf[a][i] = (uint16_t)i;
-    rk_load_keys(S, sdata, sidx, m, -1, L, isf);
-    rk_sort(S, a, t, m, idx_bits);
-    for (int d = 0; d < D; d++) {
-      rk_load_keys(S, sdata, sidx, m, d, L, isf);
-      rk_sort(S, a, t, m, 32);
-    }
+    rk_sort(S, a, t, m, rk_load_keys(S, sdata, sidx, m, -1, L, isf));
+    for (int d = 0; d < D; d++) rk_sort(S, a, t, m, rk_load_keys(S, sdata, sidx, m, d, L, isf));
     // buffers: order D-1 in a, scratch t, the others free for orders 0..D-2
     map = 0;
     {
@@ -1558,24 +1578,50 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
       int bd = rk_buf(map, d);
       const uint16_t *prev = S.buf[rk_buf(map, d == 0 ? D - 1 : d - 1)];
       for (int i = threadIdx.x; i < m; i += NT) S.buf[bd][i] = prev[i];
-      rk_load_keys(S, sdata, sidx, m, d, L, isf);
-      rk_sort(S, bd, t, m, 32);
+      rk_sort(S, bd, t, m, rk_load_keys(S, sdata, sidx, m, d, L, isf));
       map = (map & ~(15u << (4 * d))) | ((uint32_t)bd << (4 * d));
     }
   }
 
   const int p0 = threadIdx.x * E;
   uint32_t *sizes = S.kbuf;  // per level: subtree size | left-child size << 16, per block
+  // Levels whose blocks (B positions) are no wider than one wave's 64 * E
+  // positions run wave by wave: every block then lies inside one wave, so
+  // the level needs no workgroup barrier (a wave's LDS accesses are
+  // performed in order; wave_barrier keeps the compiler's order).  Waves
+  // then drift apart by whole levels, and block numbers double per level,
+  // so the per-block words (sizes, partition bases) of such levels live in
+  // a region of the wave's own: wave w's i-th block at slot w * 32 E + i
+  // (a wave holds at most 32 E blocks of >= 2 positions).
+  static_assert(32 * E <= RK_CSTRIDE && 32 * E * (NT / 64) <= CAP, "per-wave block slots");
+  const int lane = (int)threadIdx.x & 63;
+  const int wslot0 = ((int)threadIdx.x >> 6) * 32 * E;
   for (int j = 0; j < hs; j++) {
     const int lgB = hs - j;
     const int B = 1 << lgB;
+    const bool wl = B <= 64 * E;  // wave-local level
     const int first_node = (g_node + 1) * (1 << j) - 1;
     const int dj = (L0 + j) % D;
-    for (int b = threadIdx.x; b < (1 << j); b += NT) {
-      const int node = first_node + b;
-      sizes[b] = (uint32_t)subtree_size(node, n) | ((uint32_t)subtree_size(2 * node + 1, n) << 16);
+    // slot of block b's words: b itself, or its slot in the wave's region
+    const int wb0 = (p0 - lane * E) >> lgB;  // the wave's first block
+    auto bslot = [&](int b) { return wl ? wslot0 + (b - wb0) : b; };
+    if (wl) {
+      if (j == 0 || !(B * 2 <= 64 * E)) __syncthreads();  // leaving the workgroup-wide levels
+      for (int i = lane; i < (64 * E) >> lgB; i += 64) {
+        const int b = wb0 + i;
+        const int node = first_node + b;
+        if (b < (1 << j))
+          sizes[wslot0 + i] =
+              (uint32_t)subtree_size(node, n) | ((uint32_t)subtree_size(2 * node + 1, n) << 16);
+      }
+      __builtin_amdgcn_wave_barrier();
+    } else {
+      for (int b = threadIdx.x; b < (1 << j); b += NT) {
+        const int node = first_node + b;
+        sizes[b] = (uint32_t)subtree_size(node, n) | ((uint32_t)subtree_size(2 * node + 1, n) << 16);
+      }
+      __syncthreads();
     }
-    __syncthreads();
     // pivots and sides from the order of this level's dim
     {
       const uint16_t *A = S.buf[rk_buf(map, dj)];
@@ -1585,7 +1631,7 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
         const int p = p0 + e;
         if (p >= Cs) continue;
         const int b = p >> lgB, r = p & (B - 1);
-        const uint32_t zs = sizes[b];
+        const uint32_t zs = sizes[bslot(b)];
         const int sz = (int)(zs & 0xffffu);
         if (r >= sz) continue;
         const int left = (int)(zs >> 16);
@@ -1598,7 +1644,8 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
           T[r < left ? p : (b << lgB) + (B >> 1) + (r - left - 1)] = sl;
         }
       }
-      __syncthreads();
+      if (wl) __builtin_amdgcn_wave_barrier();
+      else __syncthreads();
       const int x = rk_buf(map, dj);
       map = (map & ~(15u << (4 * dj))) | ((uint32_t)t << (4 * dj));
       t = x;
@@ -1619,7 +1666,7 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
         sl[e] = 0;
         if (p < Cs) {
           const int b = p >> lgB, r = p & (B - 1);
-          if (r < (int)(sizes[b] & 0xffffu)) sl[e] = A[p], sd[e] = 0;
+          if (r < (int)(sizes[bslot(b)] & 0xffffu)) sl[e] = A[p], sd[e] = 0;
         }
       }
 #pragma unroll
@@ -1628,28 +1675,44 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
         cntL += sd[e] == 0;
         cntR += sd[e] == 1;
       }
-      // packed (right << 16 | left) prefix over all positions
-      uint32_t pre = rk_block_scan<NT>((cntR << 16) | cntL, S.wsum);
+      // packed (right << 16 | left) prefix over the positions (of the
+      // workgroup, or of the wave on a wave-local level: a block's
+      // positions are consecutive either way)
+      uint32_t pre;
+      const uint32_t packed = (cntR << 16) | cntL;
+      if (wl) {
+        uint32_t incl = packed;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t u = (uint32_t)__shfl_up((int)incl, o);
+          if (lane >= o) incl += u;
+        }
+        pre = incl - packed;
+      } else {
+        pre = rk_block_scan<NT>(packed, S.wsum);
+      }
       uint32_t at[E];
 #pragma unroll
       for (int e = 0; e < E; e++) {
         at[e] = pre;
         pre += (sd[e] == 0) ? 1u : ((sd[e] == 1) ? (1u << 16) : 0u);
         const int p = p0 + e;
-        if (p < Cs && (p & (B - 1)) == 0) S.cnt[p >> lgB] = at[e];
+        if (p < Cs && (p & (B - 1)) == 0) S.cnt[bslot(p >> lgB)] = at[e];
       }
-      __syncthreads();
+      if (wl) __builtin_amdgcn_wave_barrier();
+      else __syncthreads();
 #pragma unroll
       for (int e = 0; e < E; e++) {
         if (sd[e] > 1) continue;
         const int p = p0 + e;
         const int b = p >> lgB;
-        const uint32_t b0 = S.cnt[b];
+        const uint32_t b0 = S.cnt[bslot(b)];
         const int np = sd[e] == 0 ? (b << lgB) + (int)((at[e] & 0xffffu) - (b0 & 0xffffu))
                                   : (b << lgB) + (B >> 1) + (int)((at[e] >> 16) - (b0 >> 16));
         T[np] = sl[e];
       }
-      __syncthreads();
+      if (wl) __builtin_amdgcn_wave_barrier();
+      else __syncthreads();
       const int x = rk_buf(map, d);
       map = (map & ~(15u << (4 * d))) | ((uint32_t)t << (4 * d));
       t = x;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdlib>
 
 thread_local emu_uint3 threadIdx, blockIdx;
 emu_uint3 blockDim, gridDim;
@@ -207,8 +208,13 @@ void launch_impl(dim3 grid, dim3 block, size_t lds, const std::function<void()> 
             }
           if (live == 0) break;
           bool released = false;
-          // a wave whose live lanes all wait at one wave intrinsic
-          for (int w = 0; w < nw; w++) {
+          // a wave whose live lanes all wait at one wave intrinsic.  With
+          // EMU_SKEW=1 (2) only the lowest (highest) such wave is released,
+          // so that it runs on to the next workgroup barrier before any
+          // other wave moves: races between waves that drift apart show.
+          static const int skew = getenv("EMU_SKEW") ? atoi(getenv("EMU_SKEW")) : 0;
+          for (int wi = 0; wi < nw && !(skew && released); wi++) {
+            const int w = skew == 2 ? nw - 1 - wi : wi;
             int op = -1;
             bool all = true, any = false;
             uint64_t mask = 0;
