@@ -266,7 +266,9 @@ def serve_bench(args, ck, datagen, torch, np):
     rate = float(args.serve_rate)
     arrivals = np.cumsum(expo / rate)
     b_ms, b_wall, b_res, b_batches = replay(0, arrivals, R)
-    npr = int(min(R, max(50, rate * 1.0)))
+    # the same trace, every request, one launch sequence each: both tails
+    # are taken over the identical requests
+    npr = R
     p_ms, p_wall, p_res, p_batches = replay(1, arrivals, npr)
     same = all(np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
                for a, c in zip(b_res[:npr], p_res))
