@@ -138,15 +138,6 @@ __device__ inline E tree_at(const TreeSet<E> &set, int i)
   return set.tab ? set.tab[i] : set.one;
 }
 
-// ---------------------------------------------------------------------------
-// zero fill (used instead of hipMemsetAsync so the sequence can be captured)
-// ---------------------------------------------------------------------------
-__global__ void k_zero(uint32_t *p, size_t n)
-{
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    p[i] = 0u;
-}
 
 // ---------------------------------------------------------------------------
 // world bounds (builder_common.h:76-163): block min/max of order-preserving
@@ -226,18 +217,61 @@ __global__ void k_bounds_final(TreeSet<InTree> set, int ntrees, const uint32_t *
   }
 }
 
+// Counts bin hb (if has) in the LDS histogram h: one atomic per wave when
+// every counted lane hits the same bin (the common case deep in the tree,
+// where a subtree's keys share their high digits).  Every lane of the wave
+// calls it.
+__device__ inline void lds_count(uint32_t *h, bool has, uint32_t hb)
+{
+  const int lane = (int)threadIdx.x & 63;
+  const uint64_t act = __ballot(has);
+  if (!act) return;
+  const int first = __ffsll((unsigned long long)act) - 1;
+  const uint32_t fb = __shfl(hb, first);
+  const uint64_t same = __ballot(has && hb == fb);
+  if (same == act) {
+    if (lane == first) atomicAdd(&h[fb], (uint32_t)__popcll(act));
+  } else if (has) {
+    atomicAdd(&h[hb], 1u);
+  }
+}
+
+// Adds a tile's LDS digit histogram s_h[0..nbins) to a global digit
+// histogram row h with atomics: the non-empty bins, and one bitmap bit per
+// bin (32 bins a word, after the DIGIT_MAX_BINS counters).
+__device__ inline void flush_hist(uint32_t *h, const uint32_t *s_h, int nbins)
+{
+  const int lane = (int)threadIdx.x & 63;
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
+    const uint32_t c = s_h[b];
+    if (c) atomicAdd(&h[b], c);
+    // bins b..: lanes 0-31 and 32-63 of a wave cover two bitmap words
+    const uint64_t m = __ballot(c != 0u);
+    const uint32_t part = (lane < 32) ? (uint32_t)m : (uint32_t)(m >> 32);
+    if ((lane & 31) == 0 && part) atomicOr(&h[DIGIT_MAX_BINS + (b >> 5)], part);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // copy-in: working buffer A gets the points, their input index inside their
 // tree, and the level-0 key.
 // ---------------------------------------------------------------------------
+// Each tile also clears its share of the select's work area (digit
+// histograms, cursors, tie bounds: zwords words from zero), which the
+// levels' first launches need zeroed.  (A fill kernel rather than
+// hipMemsetAsync, so that a forest build can be captured into a graph.)
 __global__ __launch_bounds__(256) void k_copy_in(TreeSet<InTree> set, uint8_t *data,
                                                  uint32_t *idx, uint32_t *key,
                                                  const uint8_t *points, cukd_layout L,
-                                                 const SegState *st, int explicit_dim)
+                                                 const SegState *st, int explicit_dim,
+                                                 uint32_t *zero, size_t zwords)
 {
   __shared__ InTree s_tree;
   if (threadIdx.x == 0)
     s_tree = find_tree(set, (int)blockIdx.x, [](const InTree &x) { return x.tile0; });
+  for (size_t z = (size_t)blockIdx.x * blockDim.x + threadIdx.x; z < zwords;
+       z += (size_t)gridDim.x * blockDim.x)
+    zero[z] = 0u;
   __syncthreads();
   const InTree tr = s_tree;
   const int d0 = explicit_dim ? (int)st[tr.seg].dim : 0;
@@ -259,8 +293,37 @@ __global__ __launch_bounds__(256) void k_copy_in(TreeSet<InTree> set, uint8_t *d
 // global levels
 // ---------------------------------------------------------------------------
 
-// Per-level reset of the subtree states; round-robin dims and chains are
-// closed forms of the level.  One thread per segment of the level.
+// The start of subtree s of a level (histogram slot hs = the tree's slot
+// plus s): its select state, partition cursors and tie-word bounds.
+// Round-robin dims and chains are closed forms of the level; explicit
+// dims and domains are written by the level above (or the root's bounds).
+__device__ inline void init_state(SegState &t, int level, int s, int n, int D, bool expl,
+                                  uint32_t *cursors, uint32_t *tb, int hs)
+{
+  const int node = (1 << level) - 1 + s;
+  t.krem = (uint32_t)subtree_size(2 * node + 1, n);
+  t.word = 0;
+  t.prefix = 0;
+  t.resolved = 0;
+  t.tiles_done = 0;
+  t.gcount = 0;
+  if (!expl) {
+    t.dim = (uint32_t)(level % D);
+    uint32_t chain = 0, len = 0;
+    for (int j = level; j >= 0 && (int)len < D; j--) chain |= (uint32_t)(j % D) << (4 * len++);
+    t.chain = chain;
+    t.chain_len = len;
+  }
+  cursors[2 * hs] = 0u;
+  cursors[2 * hs + 1] = 0u;
+  for (int w = 0; w < MAX_WORDS; w++) {
+    tb[(size_t)hs * 2 * MAX_WORDS + w] = 0xffffffffu;
+    tb[(size_t)hs * 2 * MAX_WORDS + MAX_WORDS + w] = 0u;
+  }
+}
+
+// Start of level 0 (later levels start in the tie launch of the level
+// above).  One thread per segment of the level.
 __global__ void k_level_init(TreeSet<LvlTree> set, int nsegs, SegState *st,
                              uint32_t *cursors, uint32_t *tb, int level, int D, int explicit_dim)
 {
@@ -268,27 +331,7 @@ __global__ void k_level_init(TreeSet<LvlTree> set, int nsegs, SegState *st,
   if (sg >= nsegs) return;
   const LvlTree tr = find_tree(set, sg, [](const LvlTree &x) { return x.hseg; });
   const int s = sg - tr.hseg;
-  const int node = (1 << level) - 1 + s;
-  SegState &t = st[tr.seg + s];
-  t.krem = (uint32_t)subtree_size(2 * node + 1, tr.n);
-  t.word = 0;
-  t.prefix = 0;
-  t.resolved = 0;
-  t.tiles_done = 0;
-  t.gcount = 0;
-  if (!explicit_dim) {
-    t.dim = (uint32_t)(level % D);
-    uint32_t chain = 0, len = 0;
-    for (int j = level; j >= 0 && (int)len < D; j--) chain |= (uint32_t)(j % D) << (4 * len++);
-    t.chain = chain;
-    t.chain_len = len;
-  }
-  cursors[2 * sg] = 0u;
-  cursors[2 * sg + 1] = 0u;
-  for (int w = 0; w < MAX_WORDS; w++) {
-    tb[(size_t)sg * 2 * MAX_WORDS + w] = 0xffffffffu;
-    tb[(size_t)sg * 2 * MAX_WORDS + MAX_WORDS + w] = 0u;
-  }
+  init_state(st[tr.seg + s], level, s, tr.n, D, explicit_dim != 0, cursors, tb, sg);
 }
 
 // Value of composite-key word w of an element (word 0 is the cached key).
@@ -505,7 +548,6 @@ __global__ __launch_bounds__(SELECT_NT) void k_select_digit(
   __syncthreads();
   // tiles inside one subtree use slot 0 only
   const int seg1_begin = nslots == 2 ? g.offset(seg1) : g.n_elems;
-  const int lane = (int)threadIdx.x & 63;
   uint32_t *hflat = &s_hist[0][0];
   for (int base = t0; base < t1; base += SELECT_NT * SEL_BATCH) {
     // all of a thread's key loads in flight before any is used
@@ -521,21 +563,7 @@ __global__ __launch_bounds__(SELECT_NT) void k_select_digit(
       const int slot = p >= seg1_begin ? 1 : 0;
       const uint32_t v = kv[u];
       const bool has = p < t1 && (hi_shift >= 32 || (v >> hi_shift) == prefix[slot]);
-      const uint32_t hb = (uint32_t)slot * DIGIT_MAX_BINS + ((v >> shift) & mask);
-      // one LDS atomic per wave when every counted lane hits the same bin
-      // (the common case deep in the tree, where a subtree's keys share
-      // their high digits)
-      const uint64_t act = __ballot(has);
-      if (act) {
-        const int first = __ffsll((unsigned long long)act) - 1;
-        const uint32_t fb = __shfl(hb, first);
-        const uint64_t same = __ballot(has && hb == fb);
-        if (same == act) {
-          if (lane == first) atomicAdd(&hflat[fb], (uint32_t)__popcll(act));
-        } else if (has) {
-          atomicAdd(&hflat[hb], 1u);
-        }
-      }
+      lds_count(hflat, has, (uint32_t)slot * DIGIT_MAX_BINS + ((v >> shift) & mask));
     }
   }
   __syncthreads();
@@ -543,15 +571,7 @@ __global__ __launch_bounds__(SELECT_NT) void k_select_digit(
   const int hcopy = (int)blockIdx.x & (hcopies - 1);
   for (int sl = 0; sl < nslots; sl++) {
     const int s = sl ? seg1 : seg0;
-    uint32_t *h = hd + ((size_t)(tr.hseg + s) * hcopies + hcopy) * HIST_STRIDE;
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
-      const uint32_t c = s_hist[sl][b];
-      if (c) atomicAdd(&h[b], c);
-      // bins b..: lanes 0-31 and 32-63 of a wave cover two bitmap words
-      const uint64_t m = __ballot(c != 0u);
-      const uint32_t part = (lane < 32) ? (uint32_t)m : (uint32_t)(m >> 32);
-      if ((lane & 31) == 0 && part) atomicOr(&h[DIGIT_MAX_BINS + (b >> 5)], part);
-    }
+    flush_hist(hd + ((size_t)(tr.hseg + s) * hcopies + hcopy) * HIST_STRIDE, s_hist[sl], nbins);
   }
 }
 
@@ -587,7 +607,8 @@ enum { TIE_BATCH = 8 };
 __global__ __launch_bounds__(SELECT_NT) void k_tie_resolve(
     TreeSet<LvlTree> set, int level, const uint32_t *key, const uint8_t *data,
     const uint32_t *idx, SegState *st, SegState *st_fin, const uint32_t *h2, const uint32_t *rk,
-    int hcopies, uint8_t *scratch, uint32_t *tb, cukd_layout L, int explicit_dim)
+    int hcopies, uint8_t *scratch, uint32_t *tb, uint32_t *cursors_next, uint32_t *tb_next,
+    cukd_layout L, int explicit_dim)
 {
   enum { NW = SELECT_NT / 64 };
   __shared__ uint32_t s_hist[DIGIT_MAX_BINS];
@@ -653,6 +674,12 @@ __global__ __launch_bounds__(SELECT_NT) void k_tie_resolve(
     if (g.offset(s) >= t0) {
       if (explicit_dim) write_child_domains(s_st[sl], child_st, s, L, isf);
       if (s_st[sl].resolved) st_fin[s] = s_st[sl];
+      // the children's start on the next top level (their slots there
+      // follow the tree's next-level slot)
+      if (tr.more)
+        for (int side = 0; side < 2; side++)
+          init_state(child_st[2 * s + side], level + 1, 2 * s + side, tr.n, L.num_dims,
+                     explicit_dim != 0, cursors_next, tb_next, tr.nhseg + 2 * s + side);
     }
   }
   const bool tied0 = !s_st[0].resolved;
@@ -1872,6 +1899,7 @@ int make_forest_plan(ForestPlan &P, const long long *off, int ntrees, const cukd
       x.ptile0 = ptl;
       ptl += (g.n_elems + x.tpart - 1) / x.tpart;
       x.more = l + 1 < L0[t] ? 1 : 0;
+      x.nhseg = 0;
       hseg += 1 << l;
       v.push_back(x);
     }
@@ -1880,6 +1908,28 @@ int make_forest_plan(ForestPlan &P, const long long *off, int ntrees, const cukd
     P.lvl_stiles.push_back(stl);
     P.lvl_ptiles.push_back(ptl);
     P.max_hseg = (size_t)hseg > P.max_hseg ? (size_t)hseg : P.max_hseg;
+  }
+  // each tree's slots on the next level (the trees of level l + 1 are
+  // those of level l with 'more', in the same order)
+  for (int l = 0; l + 1 < P.max_L0; l++) {
+    size_t i = 0;
+    for (LvlTree &x : P.lvl[l]) {
+      if (!x.more) continue;
+      while (P.lvl[l + 1][i].base != x.base) i++;
+      x.nhseg = P.lvl[l + 1][i].hseg;
+    }
+  }
+  // Histogram copies per subtree where many tiles add into one subtree (at
+  // least 64 tiles per copy: the later passes read every copy, and 32 or
+  // 16 tiles per copy measured slower at 10M points), as many as the
+  // buffers (sized for the level with the most subtrees) hold.
+  for (int l = 0; l < P.max_L0; l++) {
+    const int segs = P.lvl_segs[l];
+    const int tiles_per_seg = P.lvl_stiles[l] / (segs > 0 ? segs : 1);
+    int hc = 1;
+    while (hc < SEL_HCOPIES && tiles_per_seg >= 64 * hc && (size_t)segs * hc * 2 <= P.max_hseg)
+      hc *= 2;
+    P.lvl_hcopies.push_back(hc);
   }
   // finish workgroups, by capacity class
   for (int c = 0; c < 5; c++) P.fin_blocks[c] = 0;
@@ -1927,8 +1977,8 @@ int make_forest_plan(ForestPlan &P, const long long *off, int ntrees, const cukd
   P.off_hist = o; o = align_up(o + 3 * P.max_hseg * HIST_STRIDE * 4);  // digits 0, 1, 2
   P.off_stfin = o; o = align_up(o + P.max_hseg * sizeof(SegState));
   P.off_rk = o; o = align_up(o + P.max_hseg * 4 * 4);
-  P.off_cursors = o; o = align_up(o + P.max_hseg * 2 * 4);
-  P.off_tb = o; o = align_up(o + P.max_hseg * 2 * MAX_WORDS * 4);
+  P.off_cursors = o; o = align_up(o + 2 * P.max_hseg * 2 * 4);  // even and odd levels
+  P.off_tb = o; o = align_up(o + 2 * P.max_hseg * 2 * MAX_WORDS * 4);  // even and odd levels
   P.zero_end = o;
   P.off_acc = o; o = align_up(o + (size_t)ntrees * 2 * D * 4);
   if (device_tables) {
@@ -1987,19 +2037,21 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
   uint32_t *idxA = (uint32_t *)(ws + P.off_idxA), *idxB = (uint32_t *)(ws + P.off_idxB);
   uint32_t *keyA = (uint32_t *)(ws + P.off_keyA), *keyB = (uint32_t *)(ws + P.off_keyB);
   SegState *st = (SegState *)(ws + P.off_st);
+  // digit histograms 0, 1, 2; the cursors and tie-word bounds alternate
+  // between even and odd levels (a level's tie launch starts the next
+  // level's while its own are still in use)
   uint32_t *h0 = (uint32_t *)(ws + P.off_hist);
   uint32_t *h1 = h0 + P.max_hseg * HIST_STRIDE;
   uint32_t *h2 = h1 + P.max_hseg * HIST_STRIDE;
   SegState *stfin = (SegState *)(ws + P.off_stfin);
   uint32_t *rk = (uint32_t *)(ws + P.off_rk);
   uint32_t *cursors = (uint32_t *)(ws + P.off_cursors);
-  uint32_t *tb = (uint32_t *)(ws + P.off_tb);
+  uint32_t *tbp[2] = {(uint32_t *)(ws + P.off_tb), nullptr};
+  tbp[1] = tbp[0] + P.max_hseg * 2 * MAX_WORDS;
+  uint32_t *curp[2] = {cursors, cursors + P.max_hseg * 2};
   uint32_t *acc = (uint32_t *)(ws + P.off_acc);
   uint8_t *pts = (uint8_t *)d_points;
 
-  // zero the state area (states, histograms, cursors, flags)
-  hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, s, (uint32_t *)(ws + P.off_st),
-                     (P.zero_end - P.off_st) / 4);
   const TreeSet<InTree> in = make_set(P, ws, P.off_tab_in, P.in);
   if (d_bounds || expl) {
     const int nacc = P.ntrees * 2 * D;
@@ -2013,26 +2065,24 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
   }
   hipLaunchKernelGGL(k_copy_in, dim3(P.in_tiles), dim3(256), 0, s, in, dataA, idxA,
                      P.max_L0 > 0 ? keyA : (uint32_t *)nullptr, (const uint8_t *)pts, L,
-                     (const SegState *)st, expl ? 1 : 0);
+                     (const SegState *)st, expl ? 1 : 0, h0,
+                     P.max_L0 > 0 ? (P.zero_end - P.off_hist) / 4 : 0);
+  if (P.max_L0 > 0) {
+    const TreeSet<LvlTree> set0 = make_set(P, ws, P.tables ? P.off_tab_lvl[0] : 0, P.lvl[0]);
+    hipLaunchKernelGGL(k_level_init, dim3((P.lvl_segs[0] + 255) / 256), dim3(256), 0, s, set0,
+                       P.lvl_segs[0], st, curp[0], tbp[0], 0, D, expl ? 1 : 0);
+  }
   uint8_t *dcur = dataA, *dnext = dataB;
   uint32_t *icur = idxA, *inext = idxB, *kcur = keyA, *knext = keyB;
   for (int level = 0; level < P.max_L0; level++) {
     const TreeSet<LvlTree> set =
         make_set(P, ws, P.tables ? P.off_tab_lvl[level] : 0, P.lvl[level]);
-    hipLaunchKernelGGL(k_level_init, dim3((P.lvl_segs[level] + 255) / 256), dim3(256), 0, s, set,
-                       P.lvl_segs[level], st, cursors, tb, level, D, expl ? 1 : 0);
-    // histogram copies per subtree where many tiles add into one subtree
-    // (at least 64 tiles per copy: the later passes read every copy, and 32
-    // or 16 tiles per copy measured slower at 10M points), as many as the
-    // buffer (sized for the level with the most subtrees) holds
-    const int tiles_per_seg = P.lvl_stiles[level] / (P.lvl_segs[level] > 0 ? P.lvl_segs[level] : 1);
-    int hcopies = 1;
-    while (hcopies < SEL_HCOPIES && tiles_per_seg >= 64 * hcopies &&
-           (size_t)P.lvl_segs[level] * hcopies * 2 <= P.max_hseg)
-      hcopies *= 2;
+    const int par = level & 1;
+    const int hcopies = P.lvl_hcopies[level];
     // key word 0 (the split coordinate) over the whole level, one digit
     // per launch; then the last pick and the few tied subtrees (over their
-    // tie groups alone) in one launch
+    // tie groups alone) in one launch, which also starts the next level's
+    // subtrees
     for (int d = 0; d < 3; d++)
       hipLaunchKernelGGL(k_select_digit, dim3(P.lvl_stiles[level]), dim3(SELECT_NT), 0, s, set,
                          level, d, (const uint32_t *)kcur, (const SegState *)st, h0, h1, h2, rk,
@@ -2040,13 +2090,14 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
     hipLaunchKernelGGL(k_tie_resolve, dim3(P.lvl_stiles[level]), dim3(SELECT_NT), 0, s, set,
                        level, (const uint32_t *)kcur, (const uint8_t *)dcur,
                        (const uint32_t *)icur, st, stfin, (const uint32_t *)h2,
-                       (const uint32_t *)rk, hcopies, dnext, tb, L, expl ? 1 : 0);
+                       (const uint32_t *)rk, hcopies, dnext, tbp[par], curp[par ^ 1],
+                       tbp[par ^ 1], L, expl ? 1 : 0);
     const int W = L.stride >> 2;
     auto kp = W <= 3 ? k_partition<3> : (W == 4 ? k_partition<4> : k_partition<0>);
     hipLaunchKernelGGL(kp, dim3(P.lvl_ptiles[level]), dim3(SEL_NT), 0, s, set, level,
                        (const uint32_t *)kcur, (const uint8_t *)dcur, (const uint32_t *)icur,
                        (const SegState *)st, (const SegState *)stfin, dnext, inext, knext,
-                       cursors, pts, L, expl ? 1 : 0, h0, h1, h2,
+                       curp[par], pts, L, expl ? 1 : 0, h0, h1, h2,
                        P.lvl_segs[level] * hcopies);
     uint8_t *t8 = dcur; dcur = dnext; dnext = t8;
     uint32_t *t32 = icur; icur = inext; inext = t32;

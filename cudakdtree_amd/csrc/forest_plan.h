@@ -16,6 +16,7 @@ struct LvlTree {
   int seg;     // state index of the tree's segment 0 on this level
   int cseg;    // state index of its segment 0 on the next level
   int hseg;    // first histogram / cursor slot on this level
+  int nhseg;   // the same on the next level (when 'more')
   int stile0;  // first select tile
   int ptile0;  // first partition tile
   int tsel;    // select tile size
@@ -49,6 +50,7 @@ struct ForestPlan {
   int max_L0 = 0;
   std::vector<std::vector<LvlTree>> lvl;  // per top level: the trees still on it
   std::vector<int> lvl_segs, lvl_stiles, lvl_ptiles;
+  std::vector<int> lvl_hcopies;  // digit histogram copies per subtree, per top level
   std::vector<FinTree> fin[5];            // per finish capacity 512 .. 8192
   int fin_blocks[5] = {0, 0, 0, 0, 0};
   std::vector<InTree> in;

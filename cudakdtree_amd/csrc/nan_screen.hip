@@ -7,7 +7,9 @@
 // coordinates first and returns CUKD_ERR_INVALID, leaving the array as it
 // was.  The result word of each call lives in a library-owned device array
 // (one 256-byte line per slot), so no builder, the in-place one included,
-// allocates for it.
+// allocates for it.  A free slot's word is zero: it starts so, a scan only
+// sets it, and a call that found a NaN writes zero back before it frees
+// the slot.
 #include <condition_variable>
 #include <map>
 #include <mutex>
@@ -20,7 +22,6 @@ enum { NAN_SLOTS = 64, NAN_SLOT_WORDS = 64 };
 
 static __device__ __align__(256) uint32_t g_nan_slots[NAN_SLOTS * NAN_SLOT_WORDS];
 
-__global__ void k_nan_clear(uint32_t *flag) { flag[0] = 0u; }
 
 // One thread per element (grid-stride): any NaN coordinate sets the flag.
 __global__ __launch_bounds__(256) void k_nan_scan(const uint8_t *points, long long n,
@@ -95,13 +96,17 @@ int screen_nan(const void *d_points, int n, const cukd_layout &L, hipStream_t st
   const int slot = take_slot();
   uint32_t *flag = base + slot * NAN_SLOT_WORDS;
   uint32_t h = 0;
-  hipLaunchKernelGGL(k_nan_clear, dim3(1), dim3(1), 0, stream, flag);
   const long long blocks = ((long long)n + 255) / 256;
   hipLaunchKernelGGL(k_nan_scan, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0,
                      stream, (const uint8_t *)d_points, (long long)n, L, flag);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e == hipSuccess && h) {
+    const uint32_t zero = 0u;
+    e = hipMemcpyAsync(flag, &zero, 4, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  }
   give_slot(slot);
   if (e != hipSuccess) {
     set_last_error("NaN screen failed: %s", hipGetErrorString(e));
