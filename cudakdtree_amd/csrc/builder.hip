@@ -1164,12 +1164,15 @@ __global__ __launch_bounds__(SEL_NT) void k_partition(
 // LDS finish: one workgroup builds a whole subtree of at most CAP-1 nodes.
 // ---------------------------------------------------------------------------
 
+// only: if set, the workgroups whose word is zero have nothing to do (their
+// subtrees were finished by the rank finish)
 template <int CAP, int NT, bool EXPLICIT>
 __global__ __launch_bounds__(NT) void k_finish(
     TreeSet<FinTree> set, const uint8_t *dataA, const uint8_t *dataB, const uint32_t *idxA,
-    const uint32_t *idxB, const SegState *st, uint8_t *out, cukd_layout L)
+    const uint32_t *idxB, const SegState *st, uint8_t *out, cukd_layout L, const uint32_t *only)
 {
   enum { E = CAP / NT };
+  if (only && only[blockIdx.x] == 0u) return;
   __shared__ uint32_t s_xv[CAP];  // sort exchange, then the slot scatter
   __shared__ uint32_t s_split[EXPLICIT ? CAP : 1];
   __shared__ uint8_t s_sdim[EXPLICIT ? CAP : 1];
@@ -1336,10 +1339,27 @@ __global__ __launch_bounds__(NT) void k_finish(
 // every subtree from the order of its dim and stably partitions the other
 // orders by child, so the orders stay sorted inside every subtree.  Nodes,
 // pivots and tie-breaks equal k_finish's exactly.
+//
+// Explicit dims (RK_EXPLICIT): every node has its own split dim, so no
+// fixed key serves a whole level.  (RK_CHAINED applies the same scheme to
+// round-robin subtrees whose roots lie above level D-1.)  Write K(v) for the order node v's elements have
+// before v's own sort (the composite key of v's ancestors).  The kernel
+// keeps, for every dim d and block v, the order O_d|v = (c[d], K(v)).  The
+// subtree root's K is its ancestors' chain, then the input index.  Node v
+// with dim dv takes its pivot from O_dv|v; every order is then stably
+// partitioned by side, which leaves O_d|child = (c[d], K(v)) while the
+// child needs (c[d], c[dv], K(v)) = (c[d], K(child)).  The two differ only
+// inside runs of equal c[d], which a stable sort by c[dv] repairs.  Such
+// runs come from coordinate ties at the root, marked per slot and dim;
+// roots with a tie run longer than RK_RUNMAX are left to the bitonic
+// finish.  A node's dim is the widest extent of its domain, found by a
+// walk from the subtree root's box down the split table.
 // RK_CSTRIDE: one wave's 256 digit counters plus one word, so that the
 // lanes of a wave (random digits) and of the scan (same digit, successive
 // waves) spread over the LDS banks
-enum { RK_DMAX = 4, RK_CSTRIDE = 257, RK_RUNMAX = 16 };
+// RK_SLOT_BITS: the chained modes keep a node's split dim in its pivot
+// word, above the slot (CAP <= 8192).
+enum { RK_DMAX = 4, RK_CSTRIDE = 257, RK_RUNMAX = 16, RK_SLOT_BITS = 13 };
 
 template <int CAP, int NT, int DM>
 struct RankLDS {
@@ -1348,7 +1368,7 @@ struct RankLDS {
   uint32_t cnt[RK_CSTRIDE * (NT / 64)];  // radix: counts at [wave*RK_CSTRIDE + digit]; partition: subtree prefixes
   uint32_t wsum[NT / 64];
   uint32_t rlo[NT / 64], rhi[NT / 64];  // per-wave key bounds of rk_load_keys
-  uint16_t pivot[CAP];             // slot of each subtree node, by local BFS index
+  uint16_t pivot[CAP];             // slot (| split dim << RK_SLOT_BITS) of each subtree node, by local BFS index
   uint8_t side[CAP];               // per slot: 0 left, 1 right, 2 pivot
 };
 
@@ -1498,14 +1518,124 @@ __device__ inline void rk_sort(RankLDS<CAP, NT, DM> &S, int &a, int &t, int m, i
   }
 }
 
-template <int CAP, int NT, int DM>
+// Split dim of subtree node 'local' (local BFS index): the widest extent of
+// its domain, the root box clamped by the splits on the path from the root
+// (as k_finish's walk; widestDimension, box.h:110-125: ties pick the later
+// dim).  box0: the root's lower corner, then its upper corner; pv: the
+// nodes' pivot words (split dim above the slot bits).
+template <int DM>
+__device__ inline int rk_node_dim(int local, const uint32_t *box0, const uint32_t *split,
+                                  const uint16_t *pv, int D, bool isf)
+{
+  uint32_t lo[DM], hi[DM];
+#pragma unroll
+  for (int dd = 0; dd < DM; dd++) {
+    lo[dd] = dd < D ? box0[dd] : 0u;
+    hi[dd] = dd < D ? box0[D + dd] : 0u;
+  }
+  while (local > 0) {
+    const int parent = (local - 1) >> 1;
+    const int pd = pv[parent] >> RK_SLOT_BITS;
+    const uint32_t sv = split[parent];
+    const bool left = (local & 1) != 0;
+#pragma unroll
+    for (int dd = 0; dd < DM; dd++) {
+      if (dd != pd) continue;
+      if (left) {
+        const uint32_t u = hi[dd];
+        const bool lt = isf ? (__uint_as_float(u) < __uint_as_float(sv)) : ((int)u < (int)sv);
+        hi[dd] = lt ? u : sv;
+      } else {
+        const uint32_t l = lo[dd];
+        const bool lt = isf ? (__uint_as_float(sv) < __uint_as_float(l)) : ((int)sv < (int)l);
+        lo[dd] = lt ? l : sv;
+      }
+    }
+    local = parent;
+  }
+  int best = 0;
+  if (isf) {
+    float wb = 0.f;
+#pragma unroll
+    for (int dd = 0; dd < DM; dd++) {
+      if (dd >= D) break;
+      const float w = __uint_as_float(hi[dd]) - __uint_as_float(lo[dd]);
+      if (w < wb) continue;
+      wb = w;
+      best = dd;
+    }
+  } else {
+    long long wb = 0;
+#pragma unroll
+    for (int dd = 0; dd < DM; dd++) {
+      if (dd >= D) break;
+      const long long w = (long long)(int)hi[dd] - (long long)(int)lo[dd];
+      if (w < wb) continue;
+      wb = w;
+      best = dd;
+    }
+  }
+  return best;
+}
+
+// Chained modes: position q of order T holds an element that ties in
+// coordinate d with another element of the subtree (tie bit 2 + d of its
+// side byte).  If q starts a run of equal c[d] inside [lo, end), the run is
+// stably sorted by coordinate dv (insertion sort: runs are at most
+// RK_RUNMAX long).  Other positions of the run return at once.  Runs of one
+// order are disjoint, and no run's elements change their c[d], so
+// concurrent calls on one order agree on where runs start.
+__device__ inline void rk_fix_run(uint16_t *T, const uint8_t *side, int q, int lo, int end, int d,
+                                  int dv, const uint8_t *sdata, const cukd_layout &L, bool isf)
+{
+  auto key = [&](uint16_t sl, int dd) {
+    return ordered_key(load_coord_bits(sdata + (size_t)sl * L.stride, L, dd), isf);
+  };
+  const uint32_t tb = 1u << (2 + d);
+  const uint32_t kx = key(T[q], d);
+  if (q > lo) {
+    const uint16_t y = T[q - 1];
+    if ((side[y] & tb) && key(y, d) == kx) return;  // not the run's start
+  }
+  int len = 1;
+  while (q + len < end) {
+    const uint16_t y = T[q + len];
+    if (!(side[y] & tb) || key(y, d) != kx) break;
+    len++;
+  }
+  for (int i = 1; i < len; i++) {
+    const uint16_t x = T[q + i];
+    const uint32_t kv = key(x, dv);
+    int k = i;
+    while (k > 0) {
+      const uint16_t y = T[q + k - 1];
+      if (!(kv < key(y, dv))) break;
+      T[q + k] = y;
+      k--;
+    }
+    T[q + k] = x;
+  }
+}
+
+// MODE: RK_FIXED (round-robin, finish roots at L0 >= D-1), RK_EXPLICIT
+// (explicit dims), RK_CHAINED (round-robin with the explicit-dim machinery:
+// any L0, for forests whose small trees are finished from their root).
+// st: the subtree states of level L0 (explicit dims); redo: modes other
+// than RK_FIXED, one word per workgroup, set to 1 where the bitonic finish
+// must build the subtree.
+enum { RK_FIXED = 0, RK_EXPLICIT = 1, RK_CHAINED = 2 };
+template <int CAP, int NT, int DM, int MODE>
 __global__ __launch_bounds__(NT) void k_finish_rank(
     TreeSet<FinTree> set, const uint8_t *dataA, const uint8_t *dataB, const uint32_t *idxA,
-    const uint32_t *idxB, uint8_t *out, cukd_layout L)
+    const uint32_t *idxB, uint8_t *out, cukd_layout L, const SegState *st, uint32_t *redo)
 {
   enum { E = CAP / NT };
+  static_assert(CAP <= (1 << RK_SLOT_BITS), "slots fit below the dim bits");
+  // GEN: per-block dims, every order partitioned, tie runs repaired
+  constexpr bool GEN = MODE != RK_FIXED, WALK = MODE == RK_EXPLICIT;
   __shared__ RankLDS<CAP, NT, DM> S;
   __shared__ FinTree s_tree;
+  __shared__ uint32_t s_box[WALK ? 2 * RK_DMAX : 1];  // domain of the subtree root
   if (threadIdx.x == 0)
     s_tree = find_tree(set, (int)blockIdx.x, [](const FinTree &x) { return x.blk0; });
   __syncthreads();
@@ -1528,17 +1658,37 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
   const int base = L0 > 0 ? g.offset(s) : 0;
   const uint8_t *sdata = data + (size_t)base * L.stride;
   const uint32_t *sidx = idx + base;
+  // chained modes: the root's ancestor chain (chain[0] is the root's own
+  // dim; the rest, most recent first, then the input index, is the order
+  // the subtree arrives in), and for explicit dims the root's domain
+  uint32_t chain = 0;
+  int clen = 0;
+  if (WALK) {
+    const SegState &gst = st[tr.seg + s];
+    if ((int)threadIdx.x < 2 * D) s_box[threadIdx.x] = gst.box[threadIdx.x];
+    if (L0 > 0) {
+      chain = gst.chain;
+      clen = (int)gst.chain_len;
+    }
+  } else if (GEN) {
+    for (int l = L0; l >= 0 && clen < D; l--) chain |= (uint32_t)(l % D) << (4 * clen++);
+  }
 
   // Fast path: order each dim by its coordinate alone (four passes), then
   // reorder every run of equal coordinates by the rest of its key.  Runs
-  // longer than RK_RUNMAX send the subtree to the exact LSD chain below.
+  // longer than RK_RUNMAX send the subtree to the exact LSD chain below
+  // (modes other than RK_FIXED: to the bitonic finish), which also mark
+  // the run members: tie bit 2 + d of the slot's side byte.
   __shared__ int s_long;
   if (threadIdx.x == 0) s_long = 0;
   uint32_t map = 0;
   int t = DM;
   for (int d = 0; d < D; d++) {
     int a = d;
-    for (int i = threadIdx.x; i < m; i += NT) S.buf[a][i] = (uint16_t)i;
+    for (int i = threadIdx.x; i < m; i += NT) {
+      S.buf[a][i] = (uint16_t)i;
+      if (GEN && d == 0) S.side[i] = 0;
+    }
     rk_sort(S, a, t, m, rk_load_keys(S, sdata, sidx, m, d, L, isf));
     uint16_t *A = S.buf[a];
 #pragma unroll
@@ -1554,15 +1704,17 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
         s_long = 1;
         continue;
       }
-      // insertion sort of the run by (c[d-1], ..., c[d-D+1], idx)
+      // insertion sort of the run by the rest of the key: (c[d-1], ...,
+      // c[d-D+1], idx), or (chain[1], ..., idx) in the chained modes
+      const int nc = GEN ? clen : D;
       for (int i = 1; i < len; i++) {
         const uint16_t x = A[q + i];
         int k = i;
         while (k > 0) {
           const uint16_t y = A[q + k - 1];
           bool less = false, decided = false;
-          for (int c = 1; c < D && !decided; c++) {
-            const int dd = (d - c + D) % D;
+          for (int c = 1; c < nc && !decided; c++) {
+            const int dd = GEN ? (int)chain_at(chain, c) : (d - c + D) % D;
             const uint32_t kx = ordered_key(load_coord_bits(sdata + (size_t)x * L.stride, L, dd), isf);
             const uint32_t ky = ordered_key(load_coord_bits(sdata + (size_t)y * L.stride, L, dd), isf);
             if (kx != ky) {
@@ -1577,12 +1729,17 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
         }
         A[q + k] = x;
       }
+      if (GEN)
+        for (int i = 0; i < len; i++) S.side[A[q + i]] |= (uint8_t)(4u << d);
     }
     __syncthreads();
     if (s_long) break;
     map |= (uint32_t)a << (4 * d);
   }
-  if (s_long) {
+  if (GEN) {
+    if (threadIdx.x == 0) redo[blockIdx.x] = s_long ? 1u : 0u;
+    if (s_long) return;  // uniform: s_long was read after a barrier
+  } else if (s_long) {
     // exact chain.  key_{D-1}: identity, then stably by idx, c[0], ..., c[D-1]
     int a = 0;
     t = DM;
@@ -1612,6 +1769,9 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
 
   const int p0 = threadIdx.x * E;
   uint32_t *sizes = S.kbuf;  // per level: subtree size | left-child size << 16, per block
+  // chained modes: the split value of every node with grandchildren (local
+  // index < CAP/2 - 1), above the per-block words (fewer than CAP/2)
+  uint32_t *split = S.kbuf + CAP / 2;
   // Levels whose blocks (B positions) are no wider than one wave's 64 * E
   // positions run wave by wave: every block then lies inside one wave, so
   // the level needs no workgroup barrier (a wave's LDS accesses are
@@ -1620,7 +1780,7 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
   // so the per-block words (sizes, partition bases) of such levels live in
   // a region of the wave's own: wave w's i-th block at slot w * 32 E + i
   // (a wave holds at most 32 E blocks of >= 2 positions).
-  static_assert(32 * E <= RK_CSTRIDE && 32 * E * (NT / 64) <= CAP, "per-wave block slots");
+  static_assert(32 * E <= RK_CSTRIDE && 32 * E * (NT / 64) <= CAP / 2, "per-wave block slots");
   const int lane = (int)threadIdx.x & 63;
   const int wslot0 = ((int)threadIdx.x >> 6) * 32 * E;
   for (int j = 0; j < hs; j++) {
@@ -1628,29 +1788,52 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
     const int B = 1 << lgB;
     const bool wl = B <= 64 * E;  // wave-local level
     const int first_node = (g_node + 1) * (1 << j) - 1;
+    const int lvl0 = (1 << j) - 1;  // local index of the level's first node
     const int dj = (L0 + j) % D;
     // slot of block b's words: b itself, or its slot in the wave's region
     const int wb0 = (p0 - lane * E) >> lgB;  // the wave's first block
     auto bslot = [&](int b) { return wl ? wslot0 + (b - wb0) : b; };
+    auto block_words = [&](int b, int slot) {
+      const int node = first_node + b;
+      sizes[slot] = (uint32_t)subtree_size(node, n) | ((uint32_t)subtree_size(2 * node + 1, n) << 16);
+      if (WALK)
+        S.pivot[lvl0 + b] = (uint16_t)((node < n ? rk_node_dim<DM>(lvl0 + b, s_box, split, S.pivot, D, isf) : 0) << RK_SLOT_BITS);
+      else if (GEN)
+        S.pivot[lvl0 + b] = (uint16_t)(dj << RK_SLOT_BITS);
+    };
     if (wl) {
       if (j == 0 || !(B * 2 <= 64 * E)) __syncthreads();  // leaving the workgroup-wide levels
       for (int i = lane; i < (64 * E) >> lgB; i += 64) {
         const int b = wb0 + i;
-        const int node = first_node + b;
-        if (b < (1 << j))
-          sizes[wslot0 + i] =
-              (uint32_t)subtree_size(node, n) | ((uint32_t)subtree_size(2 * node + 1, n) << 16);
+        if (b < (1 << j)) block_words(b, wslot0 + i);
       }
       __builtin_amdgcn_wave_barrier();
     } else {
-      for (int b = threadIdx.x; b < (1 << j); b += NT) {
-        const int node = first_node + b;
-        sizes[b] = (uint32_t)subtree_size(node, n) | ((uint32_t)subtree_size(2 * node + 1, n) << 16);
-      }
+      for (int b = threadIdx.x; b < (1 << j); b += NT) block_words(b, b);
       __syncthreads();
     }
-    // pivots and sides from the order of this level's dim
-    {
+    // pivots and sides from the order of each block's dim (round-robin: the
+    // level's dim, whose order is partitioned on the way)
+    if (GEN) {
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        const int p = p0 + e;
+        if (p >= Cs) continue;
+        const int b = p >> lgB, r = p & (B - 1);
+        const uint32_t zs = sizes[bslot(b)];
+        if (r >= (int)(zs & 0xffffu)) continue;
+        const int left = (int)(zs >> 16);
+        const int db = S.pivot[lvl0 + b] >> RK_SLOT_BITS;
+        const uint16_t sl = S.buf[rk_buf(map, db)][p];
+        S.side[sl] = (uint8_t)((S.side[sl] & 0xfcu) | (r == left ? 2u : (r < left ? 0u : 1u)));
+        if (r == left) {
+          S.pivot[lvl0 + b] = (uint16_t)(sl | (db << RK_SLOT_BITS));
+          if (WALK && B > 2) split[lvl0 + b] = load_coord_bits(sdata + (size_t)sl * L.stride, L, db);
+        }
+      }
+      if (wl) __builtin_amdgcn_wave_barrier();
+      else __syncthreads();
+    } else {
       const uint16_t *A = S.buf[rk_buf(map, dj)];
       uint16_t *T = S.buf[t];
 #pragma unroll
@@ -1665,7 +1848,7 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
         const uint16_t sl = A[p];
         if (r == left) {
           S.side[sl] = 2;
-          S.pivot[(1 << j) - 1 + b] = sl;
+          S.pivot[lvl0 + b] = sl;
         } else {
           S.side[sl] = r < left ? 0 : 1;
           T[r < left ? p : (b << lgB) + (B >> 1) + (r - left - 1)] = sl;
@@ -1678,14 +1861,15 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
       t = x;
     }
     if (B <= 2) continue;  // the children are leaves or empty: no later level reads the orders
-    // stable partition of the other orders by side, inside each subtree
+    // stable partition of the orders by side, inside each subtree (every
+    // order in the chained modes; RK_FIXED: the others)
     for (int d = 0; d < D; d++) {
-      if (d == dj) continue;
+      if (!GEN && d == dj) continue;
       const uint16_t *A = S.buf[rk_buf(map, d)];
       uint16_t *T = S.buf[t];
       uint16_t sl[E];
       uint8_t sd[E];
-      uint32_t cntL = 0, cntR = 0;
+      uint32_t cntL = 0, cntR = 0, ties = 0;
 #pragma unroll
       for (int e = 0; e < E; e++) {
         const int p = p0 + e;
@@ -1698,7 +1882,11 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
       }
 #pragma unroll
       for (int e = 0; e < E; e++) {
-        if (sd[e] == 0) sd[e] = S.side[sl[e]];
+        if (sd[e] == 0) {
+          const uint32_t v = S.side[sl[e]];
+          sd[e] = (uint8_t)(v & 3u);
+          if (GEN && sd[e] < 2 && ((v >> (2 + d)) & 1u)) ties |= 1u << e;
+        }
         cntL += sd[e] == 0;
         cntR += sd[e] == 1;
       }
@@ -1728,6 +1916,7 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
       }
       if (wl) __builtin_amdgcn_wave_barrier();
       else __syncthreads();
+      uint16_t npos[GEN ? E : 1];
 #pragma unroll
       for (int e = 0; e < E; e++) {
         if (sd[e] > 1) continue;
@@ -1737,9 +1926,27 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
         const int np = sd[e] == 0 ? (b << lgB) + (int)((at[e] & 0xffffu) - (b0 & 0xffffu))
                                   : (b << lgB) + (B >> 1) + (int)((at[e] >> 16) - (b0 >> 16));
         T[np] = sl[e];
+        if (GEN) npos[GEN ? e : 0] = (uint16_t)np;
       }
       if (wl) __builtin_amdgcn_wave_barrier();
       else __syncthreads();
+      if (GEN && ties) {
+        // runs of equal c[d] in the children: stably by the parent's dim.
+        // (Child sizes by closed form: the next level may already be
+        // rewriting the per-block words.)
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          if (!((ties >> e) & 1u)) continue;
+          const int b = (p0 + e) >> lgB;
+          const int dv = S.pivot[lvl0 + b] >> RK_SLOT_BITS;
+          if (dv == d) continue;
+          const int node = first_node + b;
+          const int right = sd[e];
+          const int c0 = (b << lgB) + (right ? (B >> 1) : 0);
+          const int csz = subtree_size(2 * node + 1 + right, n);
+          rk_fix_run(T, S.side, npos[GEN ? e : 0], c0, c0 + csz, d, dv, sdata, L, isf);
+        }
+      }
       const int x = rk_buf(map, d);
       map = (map & ~(15u << (4 * d))) | ((uint32_t)t << (4 * d));
       t = x;
@@ -1752,8 +1959,9 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
     const int j = 31 - __clz(local + 1);
     const int node = (g_node + 1) * (1 << j) - 1 + (local - ((1 << j) - 1));
     if (node < n)
-      copy_elem(out + (size_t)node * L.stride, sdata + (size_t)S.pivot[local] * L.stride, L,
-                (L0 + j) % D);
+      copy_elem(out + (size_t)node * L.stride,
+                sdata + (size_t)(S.pivot[local] & ((1u << RK_SLOT_BITS) - 1u)) * L.stride, L,
+                GEN ? (int)(S.pivot[local] >> RK_SLOT_BITS) : (L0 + j) % D);
   }
 }
 
@@ -1783,15 +1991,16 @@ inline int cap_class(int cap) { return 31 - __builtin_clz((unsigned)cap) - 9; } 
 // with enough finish workgroups to fill the GPU every tree takes the
 // largest capacity (fewest top levels); otherwise the capacity shrinks
 // until the tree alone has >= 256 finish workgroups.
-// Explicit-dim trees start from 4096-element subtrees: their bitonic finish
-// costs grow with log^2 of the subtree size, and a 10M float3 build
-// measured 9.6 ms against 11.1 ms with 8192 (one more global level).  The
-// rank finish of round-robin trees is fastest from 8192 (5.0 vs 5.2 ms).
-void tree_cap(int n, bool many, bool expl, int &cap, int &L0)
+// Trees that only the bitonic finish can build (explicit dims with D >
+// RK_DMAX) start from 4096-element subtrees: its costs grow with log^2 of
+// the subtree size, and a 10M float3 explicit-dim build measured 9.6 ms
+// against 11.1 ms with 8192 (one more global level).  The rank finish is
+// fastest from 8192 (5.0 vs 5.2 ms for round-robin trees).
+void tree_cap(int n, bool many, bool bitonic_only, int &cap, int &L0)
 {
   const int h = num_levels_for(n);
   L0 = 0;
-  for (cap = expl ? 4096 : 8192; cap >= 512; cap >>= 1) {
+  for (cap = bitonic_only ? 4096 : 8192; cap >= 512; cap >>= 1) {
     const int lg = 31 - __builtin_clz((unsigned)cap);
     L0 = h - lg > 0 ? h - lg : 0;
     if (many || (1 << L0) >= CUKD_FINISH_MIN_BLOCKS || cap == 512) break;
@@ -1807,29 +2016,40 @@ template <int CAP, int NT>
 hipError_t launch_finish(bool expl, const TreeSet<FinTree> &set, int blocks, hipStream_t s,
                          const uint8_t *dA, const uint8_t *dB, const uint32_t *iA,
                          const uint32_t *iB, const SegState *st, uint8_t *out,
-                         const cukd_layout &L)
+                         const cukd_layout &L, const uint32_t *only)
 {
   if (expl)
     hipLaunchKernelGGL((k_finish<CAP, NT, true>), dim3(blocks), dim3(NT), 0, s, set, dA, dB, iA,
-                       iB, st, out, L);
+                       iB, st, out, L, only);
   else
     hipLaunchKernelGGL((k_finish<CAP, NT, false>), dim3(blocks), dim3(NT), 0, s, set, dA, dB, iA,
-                       iB, st, out, L);
+                       iB, st, out, L, only);
   return hipGetLastError();
 }
 
+// Rank finish (chained: round-robin finish roots above level D-1); except
+// for RK_FIXED it is followed by the bitonic finish of the subtrees it
+// flagged in 'redo' (long coordinate tie runs).
 template <int CAP, int NT>
-hipError_t launch_finish_rank(const TreeSet<FinTree> &set, int blocks, hipStream_t s,
+hipError_t launch_finish_rank(bool expl, bool chained, const TreeSet<FinTree> &set, int blocks, hipStream_t s,
                               const uint8_t *dA, const uint8_t *dB, const uint32_t *iA,
-                              const uint32_t *iB, uint8_t *out, const cukd_layout &L)
+                              const uint32_t *iB, const SegState *st, uint8_t *out,
+                              const cukd_layout &L, uint32_t *redo)
 {
-  if (L.num_dims <= 3)
-    hipLaunchKernelGGL((k_finish_rank<CAP, NT, 3>), dim3(blocks), dim3(NT), 0, s, set, dA, dB,
-                       iA, iB, out, L);
-  else
-    hipLaunchKernelGGL((k_finish_rank<CAP, NT, RK_DMAX>), dim3(blocks), dim3(NT), 0, s, set, dA,
-                       dB, iA, iB, out, L);
-  return hipGetLastError();
+  const int mode = expl ? RK_EXPLICIT : (chained ? RK_CHAINED : RK_FIXED);
+  const bool d3 = L.num_dims <= 3;
+#define CUKD_RANK_LAUNCH(DMv, MODEv)                                                     \
+  hipLaunchKernelGGL((k_finish_rank<CAP, NT, DMv, MODEv>), dim3(blocks), dim3(NT), 0, s, set, \
+                     dA, dB, iA, iB, out, L, st, redo)
+  switch (mode) {
+    case RK_EXPLICIT: if (d3) CUKD_RANK_LAUNCH(3, RK_EXPLICIT); else CUKD_RANK_LAUNCH(RK_DMAX, RK_EXPLICIT); break;
+    case RK_CHAINED: if (d3) CUKD_RANK_LAUNCH(3, RK_CHAINED); else CUKD_RANK_LAUNCH(RK_DMAX, RK_CHAINED); break;
+    default: if (d3) CUKD_RANK_LAUNCH(3, RK_FIXED); else CUKD_RANK_LAUNCH(RK_DMAX, RK_FIXED); break;
+  }
+#undef CUKD_RANK_LAUNCH
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess || mode == RK_FIXED) return e;
+  return launch_finish<CAP, NT>(expl, set, blocks, s, dA, dB, iA, iB, st, out, L, redo);
 }
 
 }  // namespace
@@ -1851,7 +2071,7 @@ int make_forest_plan(ForestPlan &P, const long long *off, int ntrees, const cukd
   for (int t = 0; t < ntrees; t++) {
     const int n = (int)(off[t + 1] - off[t]);
     if (n < 1) continue;
-    tree_cap(n, true, P.expl, cap[t], L0[t]);
+    tree_cap(n, true, P.expl && D > RK_DMAX, cap[t], L0[t]);
     blocks_max_cap += 1ll << L0[t];
   }
   const bool many = blocks_max_cap >= CUKD_FOREST_MANY_BLOCKS;
@@ -1859,7 +2079,7 @@ int make_forest_plan(ForestPlan &P, const long long *off, int ntrees, const cukd
   for (int t = 0; t < ntrees; t++) {
     const int n = (int)(off[t + 1] - off[t]);
     if (n < 1) continue;
-    tree_cap(n, many, P.expl, cap[t], L0[t]);
+    tree_cap(n, many, P.expl && D > RK_DMAX, cap[t], L0[t]);
     P.max_L0 = L0[t] > P.max_L0 ? L0[t] : P.max_L0;
   }
   // segment states, numbered level by level
@@ -1981,6 +2201,11 @@ int make_forest_plan(ForestPlan &P, const long long *off, int ntrees, const cukd
   P.off_tb = o; o = align_up(o + 2 * P.max_hseg * 2 * MAX_WORDS * 4);  // even and odd levels
   P.zero_end = o;
   P.off_acc = o; o = align_up(o + (size_t)ntrees * 2 * D * 4);
+  // explicit-dim rank finish: one flag per finish workgroup (each class
+  // launches in turn, so the largest class sets the size)
+  int max_fin = 0;
+  for (int c = 0; c < 5; c++) max_fin = P.fin_blocks[c] > max_fin ? P.fin_blocks[c] : max_fin;
+  P.off_redo = o; o = align_up(o + (size_t)max_fin * 4);
   if (device_tables) {
     P.off_tab_in = o; o = align_up(o + P.in.size() * sizeof(InTree));
     for (int c = 0; c < 5; c++) { P.off_tab_fin[c] = o; o = align_up(o + P.fin[c].size() * sizeof(FinTree)); }
@@ -2050,6 +2275,7 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
   tbp[1] = tbp[0] + P.max_hseg * 2 * MAX_WORDS;
   uint32_t *curp[2] = {cursors, cursors + P.max_hseg * 2};
   uint32_t *acc = (uint32_t *)(ws + P.off_acc);
+  uint32_t *redo = (uint32_t *)(ws + P.off_redo);
   uint8_t *pts = (uint8_t *)d_points;
 
   const TreeSet<InTree> in = make_set(P, ws, P.off_tab_in, P.in);
@@ -2108,25 +2334,28 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
     if (P.fin_blocks[c] == 0) continue;
     const TreeSet<FinTree> fs = make_set(P, ws, P.tables ? P.off_tab_fin[c] : 0, P.fin[c]);
     const int b = P.fin_blocks[c];
-    // round-robin trees whose finish subtrees see all D dims above them
+    // the rank finish for D <= RK_DMAX: fixed orders where every
+    // round-robin finish root sees all D dims above it, the chained form
+    // otherwise
     int min_L0 = 1 << 30;
     for (const FinTree &f : P.fin[c]) min_L0 = f.L0 < min_L0 ? f.L0 : min_L0;
-    if (!expl && D <= RK_DMAX && min_L0 >= D - 1) {
+    const bool chained = !expl && min_L0 < D - 1;
+    if (D <= RK_DMAX) {
       switch (c) {
-        case 0: e = launch_finish_rank<512, 256>(fs, b, s, dataA, dataB, idxA, idxB, pts, L); break;
-        case 1: e = launch_finish_rank<1024, 256>(fs, b, s, dataA, dataB, idxA, idxB, pts, L); break;
-        case 2: e = launch_finish_rank<2048, 512>(fs, b, s, dataA, dataB, idxA, idxB, pts, L); break;
-        case 3: e = launch_finish_rank<4096, 512>(fs, b, s, dataA, dataB, idxA, idxB, pts, L); break;
-        default: e = launch_finish_rank<8192, 1024>(fs, b, s, dataA, dataB, idxA, idxB, pts, L); break;
+        case 0: e = launch_finish_rank<512, 256>(expl, chained, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, redo); break;
+        case 1: e = launch_finish_rank<1024, 256>(expl, chained, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, redo); break;
+        case 2: e = launch_finish_rank<2048, 512>(expl, chained, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, redo); break;
+        case 3: e = launch_finish_rank<4096, 512>(expl, chained, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, redo); break;
+        default: e = launch_finish_rank<8192, 1024>(expl, chained, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, redo); break;
       }
       continue;
     }
     switch (c) {
-      case 0: e = launch_finish<512, 256>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
-      case 1: e = launch_finish<1024, 256>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
-      case 2: e = launch_finish<2048, 512>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
-      case 3: e = launch_finish<4096, 512>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
-      default: e = launch_finish<8192, 1024>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L); break;
+      case 0: e = launch_finish<512, 256>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, nullptr); break;
+      case 1: e = launch_finish<1024, 256>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, nullptr); break;
+      case 2: e = launch_finish<2048, 512>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, nullptr); break;
+      case 3: e = launch_finish<4096, 512>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, nullptr); break;
+      default: e = launch_finish<8192, 1024>(expl, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, nullptr); break;
     }
   }
   if (e != hipSuccess) {

@@ -59,7 +59,7 @@ struct ForestPlan {
   // workspace layout (bytes)
   size_t off_dataA = 0, off_idxA = 0, off_keyA = 0, off_dataB = 0, off_idxB = 0, off_keyB = 0;
   size_t off_st = 0, off_hist = 0, off_stfin = 0, off_rk = 0, off_cursors = 0, off_tb = 0,
-         zero_end = 0, off_acc = 0;
+         zero_end = 0, off_acc = 0, off_redo = 0;
   size_t off_tab_in = 0, off_tab_fin[5] = {0, 0, 0, 0, 0};
   std::vector<size_t> off_tab_lvl;
   size_t total = 0;

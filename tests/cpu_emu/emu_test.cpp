@@ -54,6 +54,7 @@ std::vector<uint32_t> points(int n, const cukd_layout &L, uint64_t seed, const c
       else if (!strcmp(dist, "dups")) f = 0.5f;
       else if (!strcmp(dist, "twovals")) f = (float)(r & 1);
       else if (!strcmp(dist, "coarse")) f = (float)(r % 4096) / 4096.f;
+      else if (!strcmp(dist, "fine")) f = (float)(r % 16384) / 16384.f;  // short tie runs
       else f = (float)(r >> 40) * (1.f / 16777216.f);
       uint32_t b;
       if (L.scalar_type == CUKD_SCALAR_F32) memcpy(&b, &f, 4);
@@ -285,6 +286,23 @@ int main(int argc, char **argv)
                      [=] { return build_case("uniform", n, ppp, CUKD_BUILDER_ID_DEFAULT, 23 + n); }});
   cases.push_back({"build explicit-dim photon ties n=5000",
                    [=] { return build_case("ties", 5000, photon, CUKD_BUILDER_ID_DEFAULT, 29); }});
+  // short coordinate tie runs: the rank finish's chained modes repair them
+  // level by level (explicit dims; round-robin subtrees whose roots lie
+  // above level D-1, which the bigcap build gives n < 16384)
+  cases.push_back({"build explicit-dim payload fine n=20000",
+                   [=] { return build_case("fine", 20000, ppp, CUKD_BUILDER_ID_DEFAULT, 91); }});
+  cases.push_back({"build explicit-dim float4 payload fine n=9000", [=] {
+                     const cukd_layout ppp4{4, CUKD_SCALAR_F32, 24, 0, 20, 4};
+                     return build_case("fine", 9000, ppp4, CUKD_BUILDER_ID_DEFAULT, 96);
+                   }});
+  cases.push_back({"build explicit-dim photon fine n=7000",
+                   [=] { return build_case("fine", 7000, photon, CUKD_BUILDER_ID_DEFAULT, 92); }});
+  cases.push_back({"build default float3 fine n=12000",
+                   [=] { return build_case("fine", 12000, f3, CUKD_BUILDER_ID_DEFAULT, 93); }});
+  cases.push_back({"build default float2 fine n=3000",
+                   [=] { return build_case("fine", 3000, f2, CUKD_BUILDER_ID_DEFAULT, 94); }});
+  cases.push_back({"build default int4 fine n=12000",
+                   [=] { return build_case("fine", 12000, i4, CUKD_BUILDER_ID_DEFAULT, 95); }});
   cases.push_back({"build bitonic float3 ties n=5000",
                    [=] { return build_case("ties", 5000, f3, CUKD_BUILDER_ID_BITONIC, 31); }});
   cases.push_back({"build in-place float3 uniform n=5000",

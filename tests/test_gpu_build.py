@@ -241,6 +241,18 @@ def test_explicit_dims_large(kind, n):
     check_kdtree_valid(coords_of(got, lay), dims_of(got, lay), n)
 
 
+@pytest.mark.parametrize("D", [2, 3, 4])
+@pytest.mark.parametrize("kind", ["explicit", "explicit_int", "photon"])
+def test_explicit_dims_short_tie_runs(D, kind):
+    """Explicit-dim trees over a 2^-14 grid (short runs of equal
+    coordinates; int coordinates: runs of about 16, partly beyond the rank
+    finish's run limit): the finish repairs the runs level by level by the
+    parent's split dim, or hands the subtree to the bitonic finish."""
+    coords = distribution("coarse", 300_000, D, seed=31 + D, scalar_float=kind != "explicit_int")
+    lay, got = compare(coords, kind)
+    check_kdtree_valid(coords_of(got, lay), dims_of(got, lay), len(coords))
+
+
 def test_explicit_dims_ten_million():
     """PointPlusPayload {float3; int payload; int dim} (20 bytes) at 10M:
     the default builder equals the bitonic builder (which is exact against

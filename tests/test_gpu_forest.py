@@ -38,11 +38,17 @@ def forest_points(sizes, D, seed, kind="float", dist="uniform"):
 
 
 @pytest.mark.parametrize("kind", ["float", "payload", "explicit", "photon"])
-def test_forest_build_matches_oracle_per_tree(kind):
+@pytest.mark.parametrize("dist", ["default", "coarse"])
+def test_forest_build_matches_oracle_per_tree(kind, dist):
+    """dist 'coarse': short runs of equal coordinates, which the rank
+    finish repairs level by level where a tree's finish roots lie above
+    level D-1 (small trees) or split dims are explicit."""
     D = 3
     sizes = log_uniform_sizes(60, 1, 40000, 3, empty_every=11)
     off = offsets_of(sizes)
-    coords = forest_points(sizes, D, 5, dist="ties" if kind == "float" else "uniform")
+    if dist == "default":
+        dist = "ties" if kind == "float" else "uniform"
+    coords = forest_points(sizes, D, 5, dist=dist)
     lay = layout_for(kind, D)
     elems = make_elements(coords, lay)
     f = ck.Forest(off, lay)
@@ -62,11 +68,11 @@ def test_forest_build_matches_oracle_per_tree(kind):
     f.close()
 
 
-def _finish_blocks_at_largest_cap(sizes, explicit):
+def _finish_blocks_at_largest_cap(sizes, D, explicit):
     """Finish workgroups of a forest if every tree took the largest LDS
-    capacity (the forest plan's rule: 4096 for explicit dims, 8192
-    otherwise); 512 or more puts every tree on that capacity."""
-    cap = 4096 if explicit else 8192
+    capacity (the forest plan's rule: 4096 for explicit dims with D > 4,
+    8192 otherwise); 512 or more puts every tree on that capacity."""
+    cap = 4096 if explicit and D > 4 else 8192
     total = 0
     for n in sizes:
         if n < 1:
@@ -77,15 +83,16 @@ def _finish_blocks_at_largest_cap(sizes, explicit):
 
 
 @pytest.mark.parametrize("kind", ["explicit", "photon"])
-def test_explicit_forest_with_many_finish_blocks(kind):
+@pytest.mark.parametrize("dist", ["gmm", "coarse"])
+def test_explicit_forest_with_many_finish_blocks(kind, dist):
     """An explicit-dim forest large enough (>= 512 finish workgroups) that
-    every tree finishes from 4096-element subtrees: each tree equals the
+    every tree finishes from 8192-element subtrees: each tree equals the
     oracle's buildTree_host."""
     D = 3
-    sizes = log_uniform_sizes(40, 20000, 200000, 61)
-    assert _finish_blocks_at_largest_cap(sizes, True) >= 512
+    sizes = log_uniform_sizes(40, 40000, 300000, 61)
+    assert _finish_blocks_at_largest_cap(sizes, D, True) >= 512
     off = offsets_of(sizes)
-    coords = forest_points(sizes, D, 63, dist="gmm")
+    coords = forest_points(sizes, D, 63, dist=dist)
     lay = layout_for(kind, D)
     elems = make_elements(coords, lay)
     f = ck.Forest(off, lay)
