@@ -595,6 +595,36 @@ def main():
     tree.copy_(default_tree)
     del default_tree
 
+    # explicit split dims (widest extent per node): PointPlusPayload
+    # {float3 point; int payload; int dim}, 20 bytes, over the same points
+    # with payload = input index; compared with the bitonic builder's tree
+    lay_x = ck.point_plus_payload(3, explicit_dim=True)
+    px = torch.zeros((n, 5), dtype=torch.int32, device=dev)
+    px[:, :3] = pts.view(torch.int32).reshape(n, 3)
+    px[:, 3] = torch.arange(n, dtype=torch.int32, device=dev)
+    tx = torch.empty_like(px)
+    wbx = ck.bounds_tensor(lay_x)
+    expl_ms = []
+    for i in range(args.build_steps + 1):
+        tx.copy_(px)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ck.buildTree(tx, lay_x, wbx)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        if i > 0:
+            expl_ms.append(e0.elapsed_time(e1))
+    tb = px.clone()
+    ck.build_tree(tb, lay_x, ck.BUILDER_BITONIC, ck.bounds_tensor(lay_x))
+    torch.cuda.synchronize()
+    expl_same = bool(torch.equal(tx, tb))
+    expl_med = _pct(expl_ms, 50)
+    del px, tx, tb
+    print("[bench] build 10M explicit-dim payload: median %.3f ms, same tree as bitonic: %s"
+          % (expl_med, expl_same), file=sys.stderr, flush=True)
+
     # ---------------- kNN step (the headline value) ----------------
     ids = torch.empty((nq, k), dtype=torch.int32, device=dev)
     d2 = torch.empty((nq, k), dtype=torch.float32, device=dev)
@@ -770,6 +800,8 @@ def main():
             "build_ms_10M_bitonic": round(other_builds["bitonic"][0], 3),
             "build_ms_10M_inplace": round(other_builds["inplace"][0], 3),
             "bitonic_tree_equals_default": other_builds["bitonic"][1],
+            "build_ms_10M_explicit_payload": round(expl_med, 3),
+            "explicit_tree_equals_bitonic": expl_same,
             "knn_k8_queries_per_s": round(qps, 1),
             "knn_k8_queries_per_s_unordered": round(nq / (plain_ms / 1e3), 1),
             "knn_k8_kernel_ms_one_query_per_lane": round(one_lane_ms, 4),

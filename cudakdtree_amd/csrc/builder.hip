@@ -1679,8 +1679,15 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
   // longer than RK_RUNMAX send the subtree to the exact LSD chain below
   // (modes other than RK_FIXED: to the bitonic finish), which also mark
   // the run members: tie bit 2 + d of the slot's side byte.
+  // Chained modes also hand over subtrees where more than 1/8 of the
+  // elements tie in some dim (coarse grids): the run repairs of every
+  // level reload coordinates from memory, and at that density they cost
+  // more than the bitonic finish.
   __shared__ int s_long;
+  __shared__ uint32_t s_tied[RK_DMAX];  // chained modes: tie-run members per dim
   if (threadIdx.x == 0) s_long = 0;
+  if (GEN && threadIdx.x < RK_DMAX) s_tied[threadIdx.x] = 0u;
+  bool bail = false;
   uint32_t map = 0;
   int t = DM;
   for (int d = 0; d < D; d++) {
@@ -1729,16 +1736,20 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
         }
         A[q + k] = x;
       }
-      if (GEN)
+      if (GEN) {
         for (int i = 0; i < len; i++) S.side[A[q + i]] |= (uint8_t)(4u << d);
+        atomicAdd(&s_tied[d], (uint32_t)len);
+      }
     }
     __syncthreads();
-    if (s_long) break;
+    // uniform: both were read after the barrier
+    bail = s_long != 0 || (GEN && s_tied[d] * 8u > (uint32_t)m);
+    if (bail) break;
     map |= (uint32_t)a << (4 * d);
   }
   if (GEN) {
-    if (threadIdx.x == 0) redo[blockIdx.x] = s_long ? 1u : 0u;
-    if (s_long) return;  // uniform: s_long was read after a barrier
+    if (threadIdx.x == 0) redo[blockIdx.x] = bail ? 1u : 0u;
+    if (bail) return;
   } else if (s_long) {
     // exact chain.  key_{D-1}: identity, then stably by idx, c[0], ..., c[D-1]
     int a = 0;

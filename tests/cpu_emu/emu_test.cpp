@@ -55,6 +55,7 @@ std::vector<uint32_t> points(int n, const cukd_layout &L, uint64_t seed, const c
       else if (!strcmp(dist, "twovals")) f = (float)(r & 1);
       else if (!strcmp(dist, "coarse")) f = (float)(r % 4096) / 4096.f;
       else if (!strcmp(dist, "fine")) f = (float)(r % 16384) / 16384.f;  // short tie runs
+      else if (!strcmp(dist, "grid17")) f = (float)(r % 131072) / 131072.f;  // sparser ties
       else f = (float)(r >> 40) * (1.f / 16777216.f);
       uint32_t b;
       if (L.scalar_type == CUKD_SCALAR_F32) memcpy(&b, &f, 4);
@@ -289,6 +290,14 @@ int main(int argc, char **argv)
   // short coordinate tie runs: the rank finish's chained modes repair them
   // level by level (explicit dims; round-robin subtrees whose roots lie
   // above level D-1, which the bigcap build gives n < 16384)
+  // a few percent of the elements tied: below the hand-over density, so
+  // the rank finish repairs the runs itself
+  cases.push_back({"build explicit-dim payload grid17 n=20000",
+                   [=] { return build_case("grid17", 20000, ppp, CUKD_BUILDER_ID_DEFAULT, 97); }});
+  cases.push_back({"build explicit-dim photon grid17 n=7000",
+                   [=] { return build_case("grid17", 7000, photon, CUKD_BUILDER_ID_DEFAULT, 98); }});
+  cases.push_back({"build default float3 grid17 n=12000",
+                   [=] { return build_case("grid17", 12000, f3, CUKD_BUILDER_ID_DEFAULT, 99); }});
   cases.push_back({"build explicit-dim payload fine n=20000",
                    [=] { return build_case("fine", 20000, ppp, CUKD_BUILDER_ID_DEFAULT, 91); }});
   cases.push_back({"build explicit-dim float4 payload fine n=9000", [=] {

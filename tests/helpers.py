@@ -79,6 +79,8 @@ def distribution(name, n, d, seed, scalar_float=True):
         c = datagen.gmm(n, d, seed)
     elif name == "coarse":    # uniform on a 2^-14 grid: many short runs of equal values
         c = np.floor(datagen.uniform(n, d, seed) * 16384.0).astype(np.float32) / np.float32(16384.0)
+    elif name == "grid17":    # uniform on a 2^-17 grid: a few percent of the values tied
+        c = np.floor(datagen.uniform(n, d, seed) * 131072.0).astype(np.float32) / np.float32(131072.0)
     elif name == "unique":
         c = datagen.unique_ints(n, d, seed).astype(np.float32)
     elif name == "signed_zero":  # -0.0 and +0.0 must compare equal

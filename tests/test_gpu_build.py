@@ -243,12 +243,14 @@ def test_explicit_dims_large(kind, n):
 
 @pytest.mark.parametrize("D", [2, 3, 4])
 @pytest.mark.parametrize("kind", ["explicit", "explicit_int", "photon"])
-def test_explicit_dims_short_tie_runs(D, kind):
-    """Explicit-dim trees over a 2^-14 grid (short runs of equal
-    coordinates; int coordinates: runs of about 16, partly beyond the rank
-    finish's run limit): the finish repairs the runs level by level by the
-    parent's split dim, or hands the subtree to the bitonic finish."""
-    coords = distribution("coarse", 300_000, D, seed=31 + D, scalar_float=kind != "explicit_int")
+@pytest.mark.parametrize("dist", ["grid17", "coarse"])
+def test_explicit_dims_short_tie_runs(D, kind, dist):
+    """Explicit-dim trees over a 2^-17 grid (a few percent of the
+    coordinates in short runs of equal values: the rank finish repairs
+    them level by level by the parent's split dim) and a 2^-14 grid (dense
+    ties, and int coordinates with runs beyond the run limit: subtrees
+    handed to the bitonic finish)."""
+    coords = distribution(dist, 300_000, D, seed=31 + D, scalar_float=kind != "explicit_int")
     lay, got = compare(coords, kind)
     check_kdtree_valid(coords_of(got, lay), dims_of(got, lay), len(coords))
 

@@ -38,11 +38,12 @@ def forest_points(sizes, D, seed, kind="float", dist="uniform"):
 
 
 @pytest.mark.parametrize("kind", ["float", "payload", "explicit", "photon"])
-@pytest.mark.parametrize("dist", ["default", "coarse"])
+@pytest.mark.parametrize("dist", ["default", "grid17", "coarse"])
 def test_forest_build_matches_oracle_per_tree(kind, dist):
-    """dist 'coarse': short runs of equal coordinates, which the rank
-    finish repairs level by level where a tree's finish roots lie above
-    level D-1 (small trees) or split dims are explicit."""
+    """dist 'grid17', 'coarse': short runs of equal coordinates, which the
+    rank finish repairs level by level (or, where dense, hands to the
+    bitonic finish) where a tree's finish roots lie above level D-1 (small
+    trees) or split dims are explicit."""
     D = 3
     sizes = log_uniform_sizes(60, 1, 40000, 3, empty_every=11)
     off = offsets_of(sizes)
@@ -83,7 +84,7 @@ def _finish_blocks_at_largest_cap(sizes, D, explicit):
 
 
 @pytest.mark.parametrize("kind", ["explicit", "photon"])
-@pytest.mark.parametrize("dist", ["gmm", "coarse"])
+@pytest.mark.parametrize("dist", ["gmm", "grid17", "coarse"])
 def test_explicit_forest_with_many_finish_blocks(kind, dist):
     """An explicit-dim forest large enough (>= 512 finish workgroups) that
     every tree finishes from 8192-element subtrees: each tree equals the
