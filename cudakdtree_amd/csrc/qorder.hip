@@ -69,24 +69,23 @@ __global__ __launch_bounds__(QB_NT) void k_qbounds(uint32_t *acc, const float *q
 }
 
 // the batch box, folded from the accumulators (written by the previous
-// kernel, so plain loads see them): one word per thread, folded in LDS
+// kernel, so plain loads see them): thread t < D folds lower bound t over
+// the QB_SLOTS copies, thread D + t upper bound t
 __device__ inline void qbox_load(const uint32_t *acc, int D, float *s_lo, float *s_hi)
 {
-  __shared__ uint32_t k_lo[CUKD_MAX_DIMS], k_hi[CUKD_MAX_DIMS];
-  if (threadIdx.x < CUKD_MAX_DIMS) {
-    k_lo[threadIdx.x] = 0xffffffffu;
-    k_hi[threadIdx.x] = 0u;
-  }
-  __syncthreads();
-  for (int w = threadIdx.x; w < QB_WORDS; w += blockDim.x) {
-    const int d = w % (2 * CUKD_MAX_DIMS);
-    if (d < D) atomicMin(&k_lo[d], acc[w]);
-    else if (d >= CUKD_MAX_DIMS && d < CUKD_MAX_DIMS + D) atomicMax(&k_hi[d - CUKD_MAX_DIMS], acc[w]);
-  }
-  __syncthreads();
-  if ((int)threadIdx.x < D) {
-    s_lo[threadIdx.x] = __uint_as_float(unordered_key(k_lo[threadIdx.x], true));
-    s_hi[threadIdx.x] = __uint_as_float(unordered_key(k_hi[threadIdx.x], true));
+  const int t = (int)threadIdx.x;
+  if (t < 2 * D) {
+    const bool up = t >= D;
+    const int w0 = up ? CUKD_MAX_DIMS + (t - D) : t;
+    uint32_t v = up ? 0u : 0xffffffffu;
+#pragma unroll 8
+    for (int c = 0; c < QB_SLOTS; c++) {
+      const uint32_t x = acc[c * 2 * CUKD_MAX_DIMS + w0];
+      v = up ? max(v, x) : min(v, x);
+    }
+    const float f = __uint_as_float(unordered_key(v, true));
+    if (up) s_hi[t - D] = f;
+    else s_lo[t] = f;
   }
   __syncthreads();
 }

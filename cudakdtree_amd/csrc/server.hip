@@ -6,10 +6,12 @@
 // thread that waits for its batch and completes its requests, so batches
 // run side by side on the device: a batch held up by a few long traversals
 // does not hold up the ones admitted after it.  Inside a batch, light
-// requests are launched before heavy ones (many queries times k), one
-// launch per group, and each group's requests complete as soon as their
-// own launch has run, so a light request never waits for a heavy one it
-// shares a batch with; a request larger than a slot is served in pieces.
+// requests share one launch per query class and go first; the heavy ones
+// (many queries times k) follow, cheapest first, packed into launches of
+// bounded work.  Each launch's requests complete as soon as that launch
+// has run, so no request waits for much heavier ones it shares a batch
+// with, and heavy launches stay large enough to fill the device; a
+// request larger than a slot is served in pieces.
 // No counterpart in the reference, whose
 // queries are per-thread device calls; every result equals the single-tree
 // cukd_fcp / cukd_knn result for the same queries.
@@ -77,7 +79,13 @@ inline size_t class_bytes(size_t nq, size_t nseg, int k, int D)
 // is launched after the light requests of its batch.
 inline size_t request_cost(int nq, int k) { return (size_t)nq * (size_t)(k > 0 ? k : 1); }
 
-enum { SERVER_SLOTS = 8, HEAVY_COST = 1 << 16 };
+// HEAVY_GROUP_COST: most work (list slots) packed into one launch of heavy
+// requests, unless a single request is larger: about five of the largest
+// (100K queries, k = 16).  In a burst over 512 trees, one launch per heavy
+// request reached under 2x the per-request throughput and launches of at
+// most 2^21 slots 2.3-2.8x; at 2000 requests/s batches hold about one
+// request, so the bound matters for bursts only.
+enum { SERVER_SLOTS = 8, HEAVY_COST = 1 << 16, HEAVY_GROUP_COST = 1 << 23 };
 
 // one batch: free -> packed by the worker -> launched -> completed by the
 // slot's own thread -> free (state guarded by the server's mutex)
@@ -236,16 +244,19 @@ struct cukd_server {
 
 namespace {
 
-// The launch groups of a batch: one per (k, cutoff, heavy) in order of
-// first appearance, light groups before heavy ones.  Requests keep their
-// admission order inside a group.
+// The launch groups of a batch: the light requests of one (k, cutoff)
+// class share a group, in order of first appearance (admission order inside
+// it).  The heavy requests of a class, by ascending cost, fill groups of at
+// most HEAVY_GROUP_COST.  Light groups go first, then the heavy ones by
+// ascending cost.
 struct GroupKey {
   int k;
   float cutoff;
   bool heavy;
+  size_t cost;  // heavy: the group's cost so far
   bool same(const Request *r, bool h) const
   {
-    return k == r->k && memcmp(&cutoff, &r->cutoff, 4) == 0 && heavy == h;
+    return h == heavy && k == r->k && memcmp(&cutoff, &r->cutoff, 4) == 0;
   }
 };
 
@@ -258,22 +269,49 @@ int launch_batch(cukd_server *S, Slot &sl)
   const int D = S->D;
   std::vector<GroupKey> key;
   std::vector<std::vector<int>> mem;
+  std::vector<int> heavy;
   for (int i = 0; i < (int)sl.reqs.size(); i++) {
     const Request *r = sl.reqs[i];
-    const bool h = request_cost(r->nq, r->k) >= (size_t)HEAVY_COST;
+    if (request_cost(r->nq, r->k) >= (size_t)HEAVY_COST) {
+      heavy.push_back(i);
+      continue;
+    }
     size_t g = 0;
     for (; g < key.size(); g++)
-      if (key[g].same(r, h)) break;
+      if (key[g].same(r, false)) break;
     if (g == key.size()) {
-      key.push_back(GroupKey{r->k, r->cutoff, h});
+      key.push_back(GroupKey{r->k, r->cutoff, false, 0});
       mem.emplace_back();
     }
     mem[g].push_back(i);
   }
+  std::stable_sort(heavy.begin(), heavy.end(), [&](int x, int y) {
+    return request_cost(sl.reqs[x]->nq, sl.reqs[x]->k) < request_cost(sl.reqs[y]->nq, sl.reqs[y]->k);
+  });
+  const size_t light_groups = key.size();
+  for (int i : heavy) {
+    const Request *r = sl.reqs[i];
+    const size_t c = request_cost(r->nq, r->k);
+    // the class's open group (the last one made), if the request still fits
+    size_t g = key.size();
+    for (size_t j = key.size(); j > light_groups; j--)
+      if (key[j - 1].same(r, true)) {
+        if (key[j - 1].cost + c <= (size_t)HEAVY_GROUP_COST) g = j - 1;
+        break;
+      }
+    if (g == key.size()) {
+      key.push_back(GroupKey{r->k, r->cutoff, true, 0});
+      mem.emplace_back();
+    }
+    key[g].cost += c;
+    mem[g].push_back(i);
+  }
   std::vector<int> order(key.size());
   for (size_t g = 0; g < key.size(); g++) order[g] = (int)g;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int x, int y) { return !key[x].heavy && key[y].heavy; });
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    if (key[x].heavy != key[y].heavy) return !key[x].heavy;
+    return key[x].heavy && key[x].cost < key[y].cost;
+  });
   // layout: per group [queries][seg_off][seg_tree]; then per group results
   struct Lay { size_t q, so, st, ids, d2; int nq, nseg; };
   const size_t G = key.size();
@@ -302,7 +340,8 @@ int launch_batch(cukd_server *S, Slot &sl)
     sl.groups[j].members = mem[order[j]];
   }
   if (o > sl.bytes) {
-    // admission charges exactly this layout, so this is not reached
+    // admission charges at least this layout (every heavy request as a
+    // group of its own), so this is not reached
     set_last_error("batch of %zu bytes exceeds the slot of %zu", o, sl.bytes);
     return CUKD_ERR_INVALID;
   }
@@ -451,9 +490,10 @@ void worker_main(cukd_server *S)
       if (S->pending.empty()) break;  // stopping, and every request admitted
       si = free_slot();
       Slot &sl = S->slot[si];
-      // admit requests in arrival order while the batch, laid out exactly
-      // as launch_batch lays it out (one region set per launch group),
-      // fits the slot
+      // admit requests in arrival order while the batch fits the slot, laid
+      // out as launch_batch lays it out (one region set per launch group),
+      // with every heavy request charged as a group of its own: packing
+      // heavy requests together only removes padding
       struct Cls { GroupKey key; size_t nq, nseg, bytes; };
       std::vector<Cls> cls;
       size_t bytes = 0;
@@ -462,14 +502,15 @@ void worker_main(cukd_server *S)
         Request *r = S->pending.front();
         const bool heavy = request_cost(r->nq, r->k) >= (size_t)HEAVY_COST;
         size_t c = 0;
-        for (; c < cls.size(); c++)
-          if (cls[c].key.same(r, heavy)) break;
+        for (; c < cls.size() && !heavy; c++)
+          if (cls[c].key.same(r, false)) break;
+        if (heavy) c = cls.size();
         const size_t old_b = c < cls.size() ? cls[c].bytes : 0;
         const size_t nq = (c < cls.size() ? cls[c].nq : 0) + (size_t)r->nq;
         const size_t nseg = (c < cls.size() ? cls[c].nseg : 0) + 1;
         const size_t new_b = class_bytes(nq, nseg, r->k, S->D);
         if (!sl.reqs.empty() && bytes - old_b + new_b > S->slot_bytes) break;
-        if (c == cls.size()) cls.push_back(Cls{GroupKey{r->k, r->cutoff, heavy}, 0, 0, 0});
+        if (c == cls.size()) cls.push_back(Cls{GroupKey{r->k, r->cutoff, heavy, 0}, 0, 0, 0});
         cls[c].nq = nq;
         cls[c].nseg = nseg;
         cls[c].bytes = new_b;

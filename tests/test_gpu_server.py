@@ -196,16 +196,20 @@ def test_request_larger_than_a_slot_is_served_in_pieces(served_forest):
 
 
 def test_light_and_heavy_requests_in_one_batch(served_forest):
-    """Light requests and a heavy one (nq * k >= 65536) of the same class
-    queued together run as separate launch groups of one batch, light
-    first; every result equals the single-tree query."""
+    """Light requests and heavy ones (nq * k >= 65536) queued together: the
+    light requests of a class share a launch and go first, then each heavy
+    request has a launch of its own, cheapest first.  Every result equals
+    the single-tree query."""
     f, pts, off, lay = served_forest
     srv = ck.Server(f, pts, budget_bytes=256 << 20)
     reqs = [(0, datagen.gmm(10, 3, 1), 4, np.inf), (3, datagen.gmm(20000, 3, 2), 4, np.inf),
             (1, datagen.gmm(10, 3, 3), 4, np.inf), (2, datagen.gmm(20, 3, 4), 0, np.inf),
-            (4, datagen.gmm(9000, 3, 5), 16, np.inf)]
+            (4, datagen.gmm(9000, 3, 5), 16, np.inf), (5, datagen.gmm(30, 3, 6), 16, np.inf),
+            (6, datagen.gmm(17000, 3, 7), 4, np.inf)]
     got = _serve(srv, reqs)
+    st = srv.stats()
     srv.close()
+    assert st.requests_done == len(reqs)
     for (tree, q, k, cutoff), (ids, d2) in zip(reqs, got):
         eids, ed2 = _expected(pts, off, lay, tree, q, k, cutoff)
         assert np.array_equal(ids.ravel(), eids.ravel())
