@@ -1641,12 +1641,20 @@ __global__ __launch_bounds__(NT) void k_finish_rank(
   __syncthreads();
   const FinTree tr = s_tree;
   const int n = tr.n, L0 = tr.L0;
+  const int D = L.num_dims;
+  // A finish class that mixes round-robin roots above and below level D-1
+  // is launched twice: the chained launch builds the roots above, the
+  // fixed one the others (which the chained launch marks done).
+  if (MODE == RK_FIXED && L0 < D - 1) return;
+  if (MODE == RK_CHAINED && L0 >= D - 1) {
+    if (threadIdx.x == 0) redo[blockIdx.x] = 0u;
+    return;
+  }
   const uint8_t *data = ((L0 & 1) ? dataB : dataA) + (size_t)tr.base * L.stride;
   const uint32_t *idx = ((L0 & 1) ? idxB : idxA) + tr.base;
   out += (size_t)tr.base * L.stride;
 
   const bool isf = L.scalar_type == CUKD_SCALAR_F32;
-  const int D = L.num_dims;
   const int h = num_levels_for(n);
   const int hs = h - L0;
   const int Cs = 1 << hs;
@@ -2038,28 +2046,34 @@ hipError_t launch_finish(bool expl, const TreeSet<FinTree> &set, int blocks, hip
   return hipGetLastError();
 }
 
-// Rank finish (chained: round-robin finish roots above level D-1); except
-// for RK_FIXED it is followed by the bitonic finish of the subtrees it
-// flagged in 'redo' (long coordinate tie runs).
+// Rank finish of one capacity class.  Round-robin: the chained launch for
+// finish roots above level D-1 (if any), the fixed one for the others (if
+// any).  The chained forms are followed by the bitonic finish of the
+// subtrees they flagged in 'redo' (long or dense coordinate tie runs).
 template <int CAP, int NT>
-hipError_t launch_finish_rank(bool expl, bool chained, const TreeSet<FinTree> &set, int blocks, hipStream_t s,
+hipError_t launch_finish_rank(bool expl, bool any_chained, bool any_fixed,
+                              const TreeSet<FinTree> &set, int blocks, hipStream_t s,
                               const uint8_t *dA, const uint8_t *dB, const uint32_t *iA,
                               const uint32_t *iB, const SegState *st, uint8_t *out,
                               const cukd_layout &L, uint32_t *redo)
 {
-  const int mode = expl ? RK_EXPLICIT : (chained ? RK_CHAINED : RK_FIXED);
   const bool d3 = L.num_dims <= 3;
-#define CUKD_RANK_LAUNCH(DMv, MODEv)                                                     \
-  hipLaunchKernelGGL((k_finish_rank<CAP, NT, DMv, MODEv>), dim3(blocks), dim3(NT), 0, s, set, \
-                     dA, dB, iA, iB, out, L, st, redo)
-  switch (mode) {
-    case RK_EXPLICIT: if (d3) CUKD_RANK_LAUNCH(3, RK_EXPLICIT); else CUKD_RANK_LAUNCH(RK_DMAX, RK_EXPLICIT); break;
-    case RK_CHAINED: if (d3) CUKD_RANK_LAUNCH(3, RK_CHAINED); else CUKD_RANK_LAUNCH(RK_DMAX, RK_CHAINED); break;
-    default: if (d3) CUKD_RANK_LAUNCH(3, RK_FIXED); else CUKD_RANK_LAUNCH(RK_DMAX, RK_FIXED); break;
-  }
+#define CUKD_RANK_LAUNCH(MODEv)                                                              \
+  do {                                                                                       \
+    if (d3)                                                                                  \
+      hipLaunchKernelGGL((k_finish_rank<CAP, NT, 3, MODEv>), dim3(blocks), dim3(NT), 0, s,   \
+                         set, dA, dB, iA, iB, out, L, st, redo);                             \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_finish_rank<CAP, NT, RK_DMAX, MODEv>), dim3(blocks), dim3(NT), 0, \
+                         s, set, dA, dB, iA, iB, out, L, st, redo);                          \
+  } while (0)
+  const bool chained = expl || any_chained;
+  if (expl) CUKD_RANK_LAUNCH(RK_EXPLICIT);
+  else if (any_chained) CUKD_RANK_LAUNCH(RK_CHAINED);
+  if (!expl && any_fixed) CUKD_RANK_LAUNCH(RK_FIXED);
 #undef CUKD_RANK_LAUNCH
   const hipError_t e = hipGetLastError();
-  if (e != hipSuccess || mode == RK_FIXED) return e;
+  if (e != hipSuccess || !chained) return e;
   return launch_finish<CAP, NT>(expl, set, blocks, s, dA, dB, iA, iB, st, out, L, redo);
 }
 
@@ -2345,19 +2359,22 @@ int forest_build_run(const ForestPlan &P, void *d_points, void *d_bounds, uint8_
     if (P.fin_blocks[c] == 0) continue;
     const TreeSet<FinTree> fs = make_set(P, ws, P.tables ? P.off_tab_fin[c] : 0, P.fin[c]);
     const int b = P.fin_blocks[c];
-    // the rank finish for D <= RK_DMAX: fixed orders where every
-    // round-robin finish root sees all D dims above it, the chained form
-    // otherwise
-    int min_L0 = 1 << 30;
-    for (const FinTree &f : P.fin[c]) min_L0 = f.L0 < min_L0 ? f.L0 : min_L0;
-    const bool chained = !expl && min_L0 < D - 1;
+    // the rank finish for D <= RK_DMAX: fixed orders for round-robin
+    // finish roots that see all D dims above them, the chained form for
+    // the others and for explicit dims
+    int min_L0 = 1 << 30, max_L0 = -1;
+    for (const FinTree &f : P.fin[c]) {
+      min_L0 = f.L0 < min_L0 ? f.L0 : min_L0;
+      max_L0 = f.L0 > max_L0 ? f.L0 : max_L0;
+    }
+    const bool chained = min_L0 < D - 1, fixed = max_L0 >= D - 1;
     if (D <= RK_DMAX) {
       switch (c) {
-        case 0: e = launch_finish_rank<512, 256>(expl, chained, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, redo); break;
-        case 1: e = launch_finish_rank<1024, 256>(expl, chained, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, redo); break;
-        case 2: e = launch_finish_rank<2048, 512>(expl, chained, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, redo); break;
-        case 3: e = launch_finish_rank<4096, 512>(expl, chained, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, redo); break;
-        default: e = launch_finish_rank<8192, 1024>(expl, chained, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, redo); break;
+        case 0: e = launch_finish_rank<512, 256>(expl, chained, fixed, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, redo); break;
+        case 1: e = launch_finish_rank<1024, 256>(expl, chained, fixed, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, redo); break;
+        case 2: e = launch_finish_rank<2048, 512>(expl, chained, fixed, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, redo); break;
+        case 3: e = launch_finish_rank<4096, 512>(expl, chained, fixed, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, redo); break;
+        default: e = launch_finish_rank<8192, 1024>(expl, chained, fixed, fs, b, s, dataA, dataB, idxA, idxB, st, pts, L, redo); break;
       }
       continue;
     }

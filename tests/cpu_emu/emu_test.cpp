@@ -350,6 +350,12 @@ int main(int argc, char **argv)
   cases.push_back({"forest build + kNN-4", [=] {
                      return forest_case({0, 1, 700, 700, 5000, 5003, 12000}, 61);
                    }});
+  // finish roots on both sides of level D-1 in one capacity class (the
+  // bigcap build puts all three trees at 8192: L0 = 0, 2, 3): a chained
+  // and a fixed-order launch over the same workgroups
+  cases.push_back({"forest build + kNN-4 mixed finish roots", [=] {
+                     return forest_case({0, 3000, 23000, 63000}, 64);
+                   }});
   // >= 1024 sorted queries: the large-batch kernel in the bigcap build
   cases.push_back({"forest build + kNN-4 large sorted batch", [=] {
                      return forest_case({0, 1, 700, 700, 5000, 5003, 12000}, 62, 200);
