@@ -195,7 +195,7 @@ int cukd_knn_noalloc(int traversal, int list_kind, int k, const void *d_tree,
 
 /* The launch order CUKD_QUERY_SORT uses: d_order (nq int32) receives a
  * permutation of 0..nq-1 listing the queries by the Morton code of their
- * coordinates (18-bit codes over the batch's bounding box; the order of
+ * coordinates (15-bit codes over the batch's bounding box; the order of
  * queries inside one code cell is unspecified).  No counterpart in the
  * reference. */
 int cukd_query_order(const float *d_queries, int32_t nq, int32_t num_dims,

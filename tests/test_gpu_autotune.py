@@ -75,10 +75,13 @@ def test_forest_sorted_and_autotuned_queries():
 
 
 
+MORTON_BITS = 15  # csrc/qorder.hip
+
+
 def test_query_order_is_a_morton_permutation():
     """cukd_query_order returns a permutation whose queries have
-    non-decreasing Morton cells, and launching in that order changes no
-    result."""
+    non-decreasing Morton cells (50000 queries: several counting tiles),
+    and launching in that order changes no result."""
     D = 3
     q = datagen.gmm(50000, D, 11)
     dq = torch.from_numpy(q).cuda()
@@ -86,9 +89,10 @@ def test_query_order_is_a_morton_permutation():
     assert np.array_equal(np.sort(order), np.arange(len(q)))
     lo, hi = q.min(0), q.max(0)  # float32, as on the device
     t = (q - lo) / (hi - lo)
-    cell = np.clip((t * np.float32(64)).astype(np.int64), 0, 63)
+    b = MORTON_BITS // D  # bits per dim
+    cell = np.clip((t * np.float32(1 << b)).astype(np.int64), 0, (1 << b) - 1)
     code = np.zeros(len(q), np.int64)
-    for j in range(6):
+    for j in range(b):
         for d in range(D):
             code |= ((cell[:, d] >> j) & 1) << (j * D + d)
     assert np.all(np.diff(code[order]) >= 0)
