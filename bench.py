@@ -97,6 +97,36 @@ def forest_bench(args, ck, datagen, torch, np):
     print("[bench] forest: %d trees, %d points, %d queries" % (T, N, NQ), file=sys.stderr,
           flush=True)
 
+    if args.probe_forest:
+        f = ck.Forest(off, lay)
+        b_pts = torch.empty_like(orig)
+        b_ids = torch.empty((NQ, k), dtype=torch.int32, device=dev)
+        b_d2 = torch.empty((NQ, k), dtype=torch.float32, device=dev)
+        b_r = torch.empty(NQ, dtype=torch.float32, device=dev)
+
+        def step(stream=None):
+            b_pts.copy_(orig)
+            f.build(b_pts, stream=stream)
+            f.knn(b_pts, queries, k, seg_off, seg_tree, stream=stream, out=(b_ids, b_d2, b_r),
+                  flags=ck.QUERY_AUTOTUNE)
+
+        step()
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            step(torch.cuda.current_stream())
+        for i in range(3):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            graph.replay()
+            e1.record()
+            torch.cuda.synchronize()
+            print("[bench] forest graph replay %d: %.2f ms" % (i, e0.elapsed_time(e1)),
+                  file=sys.stderr, flush=True)
+        f.close()
+        return None
+
     # plain: one cukd_build_tree + one cukd_knn per tree, the loop in C++
     # (cudakdtree_amd/cli/plain_forest.cpp), autotuned launch shapes
     p_pts = torch.empty_like(orig)
@@ -496,6 +526,12 @@ def main():
                          "calibration), one build, one kNN and one FCP launch, no JSON line")
     ap.add_argument("--probe-build", action="store_true",
                     help="for kernel-trace runs: five builds of the 10M tree, no JSON line")
+    ap.add_argument("--probe-build-explicit", action="store_true",
+                    help="for kernel-trace runs: five builds of the 10M explicit-dim "
+                         "PointPlusPayload tree, no JSON line")
+    ap.add_argument("--probe-forest", action="store_true",
+                    help="for kernel-trace runs: the forest step eager once, then three "
+                         "replays of its graph, no JSON line")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -524,6 +560,25 @@ def main():
     tree = torch.empty_like(pts)
     torch.cuda.synchronize()
     print("[bench] inputs resident: %d points, %d queries" % (n, nq), file=sys.stderr, flush=True)
+
+    if args.probe_forest:
+        forest_bench(args, ck, datagen, torch, np)
+        print("[bench] probe-forest done", file=sys.stderr, flush=True)
+        return
+
+    if args.probe_build_explicit:
+        lay_x = ck.point_plus_payload(3, explicit_dim=True)
+        px = torch.zeros((n, 5), dtype=torch.int32, device=dev)
+        px[:, :3] = pts.view(torch.int32).reshape(n, 3)
+        px[:, 3] = torch.arange(n, dtype=torch.int32, device=dev)
+        tx = torch.empty_like(px)
+        wbx = ck.bounds_tensor(lay_x)
+        for _ in range(5):
+            tx.copy_(px)
+            ck.buildTree(tx, lay_x, wbx)
+        torch.cuda.synchronize()
+        print("[bench] probe-build-explicit done", file=sys.stderr, flush=True)
+        return
 
     if args.probe_build:
         for _ in range(5):
