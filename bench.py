@@ -286,10 +286,10 @@ def serve_bench(args, ck, datagen, torch, np):
 
     # saturation: every request of a burst queued at once, batched and one
     # launch sequence per request
-    nb = min(R, 1000)
+    nb = min(R, 3000)
     _, burst_wall, _, _ = replay(0, None, nb)
     cap_qps = float(nqs[:nb].sum()) / burst_wall
-    npb = min(R, 300)
+    npb = min(R, 1000)
     _, pr_burst_wall, _, _ = replay(1, None, npb)
     pr_cap_qps = float(nqs[:npb].sum()) / pr_burst_wall
     # the Poisson trace at a fixed request rate, against both

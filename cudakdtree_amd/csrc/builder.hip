@@ -14,15 +14,23 @@
 //
 //  * top levels (subtrees larger than one LDS tile): a per-level,
 //    all-subtrees-at-once radix select over the composite key (3 digit
-//    passes per 32-bit word, a word only when an earlier one ties),
-//    followed by an unordered partition into the next level's compact
-//    layout, pivots written straight to their final node slot;
-//  * bottom levels: one workgroup per subtree loads it into LDS, puts it
-//    in composite-key order once (LSD bitonic passes), then runs every
-//    remaining level as a bitonic sort of (coord, position) keys inside
-//    power-of-two aligned blocks -- the position tie-break makes each
-//    level's sort stable, which reproduces the host's tie chain -- and
-//    writes each pivot to its node slot.
+//    launches over the split coordinate, then one launch that resolves
+//    the later key words of the few subtrees tied on it, over their tie
+//    groups alone), followed by an unordered partition into the next
+//    level's compact layout, pivots written straight to their final node
+//    slot;
+//  * bottom levels: one workgroup per subtree of at most 8191 nodes.  For
+//    D <= 4 the rank finish (k_finish_rank) sorts the subtree once per dim
+//    in LDS (stable LSD radix passes, ties ordered by the rest of the key)
+//    and then only partitions those orders level by level, taking each
+//    pivot by rank; explicit split dims and subtrees rooted above level
+//    D-1 keep the orders exact by repairing tie runs level by level.
+//    Otherwise (D > 4, or long/dense coordinate ties) the bitonic finish
+//    (k_finish) puts the subtree in composite-key order and runs every
+//    level as a bitonic sort of (coord, position) keys inside power-of-two
+//    aligned blocks -- the position tie-break makes each level's sort
+//    stable, which reproduces the host's tie chain.  Both write each pivot
+//    to its node slot.
 //
 // All launches are stream-ordered, with no host synchronisation.
 #include <stdarg.h>
