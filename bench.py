@@ -8,8 +8,10 @@ Workload (the 10M-point float3 case of the builder/query scaling config):
 
 Prints one JSON line.  value = queries/s of the kNN step; the build time,
 FCP rate and the other headline figures ride along under "metrics", the
-small plain-method case and the spatial k-d tree case under "metrics.c1"
-and "metrics.spatial", the forest and serving forms under
+small plain-method case, the rest of the builder/query scaling config
+(stack-free queries, 1M and GMM trees) and the spatial k-d tree case
+under "metrics.c1", "metrics.c2" and "metrics.spatial", the forest and
+serving forms under
 "metrics.forest" and "metrics.serve".  Inputs are resident in HBM before
 any timed region starts.  "parity" records the checks this run made
 against the oracle (test infrastructure, never the measured path): the
@@ -330,6 +332,74 @@ def serve_bench(args, ck, datagen, torch, np):
     return out
 
 
+def c2_bench(args, ck, datagen, torch, np, tree, lay, queries):
+    """The rest of the builder/query scaling config: stack-free FCP and
+    kNN-8 over the 10M uniform tree, and the 1M uniform and 10M GMM (64
+    centres, sigma 0.01) trees, each with its build ms (median of 5) and
+    stack-based FCP / kNN-8 rates; always the same 1M uniform queries
+    (seed 2), Morton-ordered launches (ordering time included)."""
+    dev = queries.device
+    nq = queries.shape[0]
+    k = 8
+    ids = torch.empty((nq, k), dtype=torch.int32, device=dev)
+    d2 = torch.empty((nq, k), dtype=torch.float32, device=dev)
+    rr = torch.empty(nq, dtype=torch.float32, device=dev)
+    fi = torch.empty(nq, dtype=torch.int32, device=dev)
+    fd = torch.empty(nq, dtype=torch.float32, device=dev)
+    def rate(fn, steps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return round(nq / ((time.perf_counter() - t0) / steps), 1)
+
+    def rates(t, traversal, steps):
+        knn = lambda: ck.knn(t, lay, queries, k, traversal=traversal, out=(ids, d2, rr),
+                             flags=ck.QUERY_SORT)
+        counter = torch.zeros(1, dtype=torch.int64, device=dev)
+        ck.set_traversal_stats(counter)
+        knn()
+        torch.cuda.synchronize()
+        ck.set_traversal_stats(None)
+        return {
+            "knn_k8_queries_per_s": rate(knn, steps),
+            "knn_k8_visits_per_query": round(int(counter.item()) / nq, 2),
+            "fcp_queries_per_s": rate(lambda: ck.fcp(t, lay, queries, traversal=traversal,
+                                                     out=(fi, fd), flags=ck.QUERY_SORT), steps)}
+
+    out = {"workload": "1M uniform float3 queries (seed 2) over: the 10M uniform tree with the "
+                       "stack-free traversal; 1M uniform (seed 1) and 10M GMM (64 centres, "
+                       "sigma 0.01, seed 1) trees with the stack-based one",
+           "stack_free_10M_uniform": rates(tree, ck.STACK_FREE, max(3, args.steps // 2))}
+    stream = torch.cuda.current_stream()
+    for name, pts_h in (("uniform_1M", datagen.uniform(1_000_000, 3, 1)),
+                        ("gmm_10M", datagen.gmm(10_000_000, 3, 1))):
+        p = torch.from_numpy(pts_h).to(dev)
+        t = torch.empty_like(p)
+        ms = []
+        for i in range(6):
+            t.copy_(p)
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            ck.buildTree(t, lay)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            if i > 0:
+                ms.append(e0.elapsed_time(e1))
+        # uniform queries over the clustered tree are mostly far from every
+        # cluster and visit many nodes each: fewer timed repetitions
+        reps = 2 if name == "gmm_10M" else max(3, args.steps // 2)
+        out[name] = {"build_ms": round(_pct(ms, 50), 3), **rates(t, ck.STACK_BASED, reps)}
+        del p, t
+    print("[bench] c2: %s" % json.dumps({kk: v for kk, v in out.items() if kk != "workload"}),
+          file=sys.stderr, flush=True)
+    return out
+
+
 def c1_bench(args, ck, datagen, torch, np):
     """The plain method at its small size: a left-balanced tree over 100K
     float3 points uniform in [0,1)^3 (seed 1) built by buildTree, then FCP
@@ -520,6 +590,7 @@ def main():
                     help="Poisson request rate (requests/s) of the serving trace")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-spatial", action="store_true")
     ap.add_argument("--probe", action="store_true",
                     help="for hardware-counter runs: one 1 GiB device copy (the counters' "
@@ -767,6 +838,7 @@ def main():
     fcp_ms = (time.perf_counter() - t0) * 1e3 / args.steps
 
     # ---------------- the other configurations' legs ----------------
+    c2 = None if args.no_c2 else c2_bench(args, ck, datagen, torch, np, tree, lay, queries)
     c1 = None if args.no_c1 else c1_bench(args, ck, datagen, torch, np)
     spatial = None if args.no_spatial else spatial_bench(args, ck, datagen, torch, np)
     forest = None if args.no_forest else forest_bench(args, ck, datagen, torch, np)
@@ -865,6 +937,7 @@ def main():
             "p99_ms": serve["serve_p99_ms"] if serve else None,
             "visits_per_query": round(visits / nq, 2),
             "c1": c1,
+            "c2": c2,
             "spatial": spatial,
             "forest": forest,
             "serve": serve,
