@@ -277,49 +277,84 @@ __global__ void k_sp_classify(const SpNode *nodes, const uint32_t *pos_prim,
   left_flag[p] = left;
 }
 
-// updatePrims part 2: stable partition into the children, child boxes
-__global__ void k_sp_scatter(SpNode *nodes, const uint32_t *pos_prim, const uint32_t *pos_node,
-                             const uint32_t *left_flag, const uint32_t *left_excl,
-                             uint32_t *npos_prim, uint32_t *npos_node, const uint8_t *pts, int n,
-                             cukd_layout L)
+// updatePrims part 2: stable partition into the children, child boxes.
+// The boxes grow with one atomic per (child, word) and wave: the lanes of a
+// wave hold consecutive positions, so nearly always one or two children,
+// and per-prim atomics on the few children of the top levels serialise
+// (min and max are order-independent, so the boxes are the same).
+__global__ __launch_bounds__(256) void k_sp_scatter(SpNode *nodes, const uint32_t *pos_prim,
+                                                    const uint32_t *pos_node,
+                                                    const uint32_t *left_flag,
+                                                    const uint32_t *left_excl, uint32_t *npos_prim,
+                                                    uint32_t *npos_node, const uint8_t *pts, int n,
+                                                    cukd_layout L)
 {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t node = pos_node[p];
-  const SpNode &t = nodes[node];
-  const uint32_t prim = pos_prim[p];
-  if (t.state != SP_SPLIT) {
-    npos_prim[p] = prim;
-    npos_node[p] = node;
-    return;
+  const int D = L.num_dims;
+  const int lane = (int)threadIdx.x & 63;
+  uint32_t mychild = 0xffffffffu;  // none: past the end, or not split this round
+  int32_t e[CUKD_MAX_DIMS];
+#pragma unroll
+  for (int d = 0; d < CUKD_MAX_DIMS; d++) e[d] = 0;
+  if (p < n) {
+    const uint32_t node = pos_node[p];
+    const SpNode &t = nodes[node];
+    const uint32_t prim = pos_prim[p];
+    if (t.state != SP_SPLIT) {
+      npos_prim[p] = prim;
+      npos_node[p] = node;
+    } else {
+      const uint32_t end = t.start + t.count;
+      const uint32_t left_end =
+          end < (uint32_t)n ? left_excl[end] : left_excl[n - 1] + left_flag[n - 1];
+      const uint32_t lcount = left_end - left_excl[t.start];
+      const uint32_t lrank = left_excl[p] - left_excl[t.start];
+      uint32_t np, child;
+      if (left_flag[p]) {
+        np = t.start + lrank;
+        child = t.offset;
+      } else {
+        np = t.start + lcount + ((uint32_t)p - t.start - lrank);
+        child = t.offset + 1;
+      }
+      npos_prim[np] = prim;
+      npos_node[np] = child;
+      SpNode &c = nodes[child];
+      if (lrank == 0 && left_flag[p]) {
+        c.count = lcount;
+        c.start = t.start;
+      }
+      if (!left_flag[p] && (uint32_t)p - t.start - lrank == 0) {
+        c.count = t.count - lcount;
+        c.start = t.start + lcount;
+      }
+      mychild = child;
+#pragma unroll
+      for (int d = 0; d < CUKD_MAX_DIMS; d++)
+        if (d < D) e[d] = abox_enc(sp_coord(pts, prim, L, d));
+    }
   }
-  const uint32_t end = t.start + t.count;
-  const uint32_t left_end = end < (uint32_t)n ? left_excl[end] : left_excl[n - 1] + left_flag[n - 1];
-  const uint32_t lcount = left_end - left_excl[t.start];
-  const uint32_t lrank = left_excl[p] - left_excl[t.start];
-  uint32_t np, child;
-  if (left_flag[p]) {
-    np = t.start + lrank;
-    child = t.offset;
-  } else {
-    np = t.start + lcount + ((uint32_t)p - t.start - lrank);
-    child = t.offset + 1;
-  }
-  npos_prim[np] = prim;
-  npos_node[np] = child;
-  SpNode &c = nodes[child];
-  for (int d = 0; d < L.num_dims; d++) {
-    const int32_t e = abox_enc(sp_coord(pts, prim, L, d));
-    atomicMin(&c.lo[d], e);
-    atomicMax(&c.hi[d], e);
-  }
-  if (lrank == 0 && left_flag[p]) {
-    c.count = lcount;
-    c.start = t.start;
-  }
-  if (!left_flag[p] && (uint32_t)p - t.start - lrank == 0) {
-    c.count = t.count - lcount;
-    c.start = t.start + lcount;
+  // every lane of the wave takes part: one child per trip
+  uint64_t active = __ballot(mychild != 0xffffffffu);
+  while (active) {
+    const int leader = __ffsll((unsigned long long)active) - 1;
+    const uint32_t c = (uint32_t)__shfl((int)mychild, leader);
+    const bool peer = mychild == c;
+    active &= ~__ballot(peer);
+#pragma unroll
+    for (int d = 0; d < CUKD_MAX_DIMS; d++) {
+      if (d >= D) break;
+      int32_t lo = peer ? e[d] : 0x7fffffff, hi = peer ? e[d] : (int32_t)0x80000000;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o));
+        hi = max(hi, __shfl_xor(hi, o));
+      }
+      if (lane == leader) {
+        atomicMin(&nodes[c].lo[d], lo);
+        atomicMax(&nodes[c].hi[d], hi);
+      }
+    }
   }
 }
 
