@@ -584,7 +584,7 @@ def main():
     ap.add_argument("--forest-steps", type=int, default=2)
     ap.add_argument("--no-forest", action="store_true")
     ap.add_argument("--serve-trees", type=int, default=512)
-    ap.add_argument("--serve-requests", type=int, default=6000)
+    ap.add_argument("--serve-requests", type=int, default=12000)
     ap.add_argument("--serve-budget-mb", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=2000.0,
                     help="Poisson request rate (requests/s) of the serving trace")
