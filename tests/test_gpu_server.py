@@ -216,6 +216,28 @@ def test_light_and_heavy_requests_in_one_batch(served_forest):
         assert np.array_equal(d2.view(np.uint32).ravel(), ed2.view(np.uint32).ravel())
 
 
+def test_heavy_requests_beyond_one_launch_bound(served_forest):
+    """Six heavy kNN-16 requests of one class submitted back to back (1.6M
+    list slots each; six exceed the 2^23 slots one heavy launch takes):
+    whatever the worker admits into one batch is packed into launches of
+    bounded work, cheapest first, each completing on its own.  Every
+    result equals the single-tree query."""
+    f, pts, off, lay = served_forest
+    srv = ck.Server(f, pts, budget_bytes=2 << 30)
+    T = len(off) - 1
+    reqs = [((3 * i + 1) % T, datagen.gmm(100_000 - 7 * i, 3, 300 + i), 16, np.inf)
+            for i in range(6)]
+    reqs.insert(2, (5 % T, datagen.gmm(40, 3, 299), 16, np.inf))  # a light one among them
+    got = _serve(srv, reqs)
+    st = srv.stats()
+    srv.close()
+    assert st.requests_done == len(reqs)
+    for (tree, q, k, cutoff), (ids, d2) in zip(reqs, got):
+        eids, ed2 = _expected(pts, off, lay, tree, q, k, cutoff)
+        assert np.array_equal(ids.ravel(), eids.ravel())
+        assert np.array_equal(d2.view(np.uint32).ravel(), ed2.view(np.uint32).ravel())
+
+
 def test_budget_and_argument_errors(served_forest):
     f, pts, off, lay = served_forest
     srv = ck.Server(f, pts, budget_bytes=1 << 20)
