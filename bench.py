@@ -400,6 +400,59 @@ def c2_bench(args, ck, datagen, torch, np, tree, lay, queries):
     return out
 
 
+def lists_bench(args, ck, torch, tree, lay, queries):
+    """Candidate lists beyond the register-resident FixedCandidateList<k>
+    (k = 1/4/8/16): stack-based kNN of the 1M queries over the 10M tree,
+    Morton-ordered once outside the timed region, for the fixed list at
+    k = 16 (registers), 20 and 64 and the heap list at k = 16, 20, 50 and 64
+    (run-time k, entries in LDS).  Full-size consistency: the sorted
+    heap-64 keys equal the fixed-64 list, their first 20 the fixed-20 list,
+    whose first 16 equal the register list, and every heap row is a
+    max-heap."""
+    dev = queries.device
+    nq = queries.shape[0]
+    qs = queries[ck.query_order(queries).long()].contiguous()
+    stream = torch.cuda.current_stream()
+    res, keys, heaps = {}, {}, {}
+    for lk, kk in ((ck.LIST_FIXED, 16), (ck.LIST_FIXED, 20), (ck.LIST_FIXED, 64),
+                   (ck.LIST_HEAP, 16), (ck.LIST_HEAP, 20), (ck.LIST_HEAP, 50),
+                   (ck.LIST_HEAP, 64)):
+        name = "%s_k%d" % ("fixed" if lk == ck.LIST_FIXED else "heap", kk)
+        ids = torch.empty((nq, kk), dtype=torch.int32, device=dev)
+        d2 = torch.empty((nq, kk), dtype=torch.float32, device=dev)
+        rr = torch.empty(nq, dtype=torch.float32, device=dev)
+        fn = lambda: ck.knn(tree, lay, qs, kk, list_kind=lk, out=(ids, d2, rr),
+                            flags=ck.QUERY_PRESORTED)
+        fn()
+        torch.cuda.synchronize()
+        reps = 3
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        res[name] = {"ms": round(ms, 3), "queries_per_s": round(nq / (ms / 1e3), 1)}
+        if name in ("fixed_k16", "fixed_k20", "fixed_k64", "heap_k64"):
+            key = (d2.view(torch.int32).to(torch.int64) << 32) | (ids.to(torch.int64) & 0xFFFFFFFF)
+            (heaps if lk == ck.LIST_HEAP else keys)[kk] = key
+        del ids, d2, rr
+    h = heaps[64]
+    par = torch.arange(1, 64, device=dev).sub_(1).div_(2, rounding_mode="floor")
+    is_heap = bool((h[:, par] >= h[:, 1:]).all().item())
+    hs = torch.sort(h, dim=1).values
+    consistent = (is_heap and torch.equal(hs, keys[64]) and torch.equal(hs[:, :20], keys[20])
+                  and torch.equal(keys[20][:, :16], keys[16]))
+    out = {"workload": "stack-based kNN of 1M uniform float3 queries (seed 2, Morton-ordered "
+                       "once) over the 10M-point tree; fixed k=16 is the register list",
+           **res, "lists_consistent": consistent}
+    print("[bench] lists: %s" % json.dumps(out), file=sys.stderr, flush=True)
+    del keys, heaps, h, hs, qs
+    return out
+
+
 def c1_bench(args, ck, datagen, torch, np):
     """The plain method at its small size: a left-balanced tree over 100K
     float3 points uniform in [0,1)^3 (seed 1) built by buildTree, then FCP
@@ -591,6 +644,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-lists", action="store_true")
     ap.add_argument("--no-spatial", action="store_true")
     ap.add_argument("--probe", action="store_true",
                     help="for hardware-counter runs: one 1 GiB device copy (the counters' "
@@ -839,6 +893,7 @@ def main():
 
     # ---------------- the other configurations' legs ----------------
     c2 = None if args.no_c2 else c2_bench(args, ck, datagen, torch, np, tree, lay, queries)
+    lists = None if args.no_lists else lists_bench(args, ck, torch, tree, lay, queries)
     c1 = None if args.no_c1 else c1_bench(args, ck, datagen, torch, np)
     spatial = None if args.no_spatial else spatial_bench(args, ck, datagen, torch, np)
     forest = None if args.no_forest else forest_bench(args, ck, datagen, torch, np)
@@ -938,6 +993,7 @@ def main():
             "visits_per_query": round(visits / nq, 2),
             "c1": c1,
             "c2": c2,
+            "lists": lists,
             "spatial": spatial,
             "forest": forest,
             "serve": serve,

@@ -8,8 +8,6 @@
 #pragma once
 #pragma clang fp contract(off)
 
-#include <new>
-
 #include "cukd/knn.h"
 #include "cukd_common.h"
 
@@ -58,24 +56,109 @@ struct TreeViewD : TreeView {
   }
 };
 
-// FixedCandidateList with a run-time k (<= 64); entries live in scratch
-struct DynFixedList {
-  uint64_t entry[64];
+// The run-time-k lists (k <= 64) keep their entries in LDS, in columns:
+// entry i of the lane's query at col[i * stride], col = base + lane and
+// stride = the workgroup size, so the lanes of a wave touch consecutive
+// words.  (A per-lane array indexed at run time lives in scratch memory:
+// k = 64 took 304 ms for 1M queries over a 10M-point tree.)  The largest
+// kept key is also held in a register, so rejected candidates -- most of them
+// -- touch no LDS.
+struct LdsCol {
+  uint64_t *p;
+  int stride;
+  __device__ inline uint64_t &operator[](int i) const { return p[(size_t)i * stride]; }
+};
+
+__device__ inline uint64_t list_key(float d2, uint32_t id)
+{
+  return ((uint64_t)__float_as_uint(d2) << 32) | id;
+}
+
+// the reference's max-heap push (knn.h:456-483, as detail::heap_push) over
+// an LDS column; top = e[0]
+__device__ inline void lds_heap_push(const LdsCol &e, int k, uint64_t v, uint64_t &top)
+{
+  if (v >= top) return;
+  int pos = 0;
+  for (;;) {
+    uint64_t largest_val = ~0ull;
+    const int c0 = 2 * pos + 1;
+    int largest = k;
+    if (c0 < k) {
+      largest = c0;
+      largest_val = e[c0];
+    }
+    const int c1 = c0 + 1;
+    if (c1 < k) {
+      const uint64_t v1 = e[c1];
+      if (v1 > largest_val) {
+        largest = c1;
+        largest_val = v1;
+      }
+    }
+    if (largest == k || largest_val < v) {
+      e[pos] = v;
+      break;
+    }
+    e[pos] = largest_val;
+    pos = largest;
+  }
+  top = e[0];
+}
+
+// in-place heapsort of a max-heap column into ascending order
+__device__ inline void lds_heap_sort(const LdsCol &e, int k)
+{
+  for (int end = k - 1; end > 0; end--) {
+    const uint64_t v = e[end];
+    e[end] = e[0];
+    int pos = 0;
+    for (;;) {
+      int c = 2 * pos + 1;
+      if (c >= end) break;
+      uint64_t cv = e[c];
+      if (c + 1 < end) {
+        const uint64_t v1 = e[c + 1];
+        if (v1 > cv) {
+          c++;
+          cv = v1;
+        }
+      }
+      if (!(cv > v)) break;
+      e[pos] = cv;
+      pos = c;
+    }
+    e[pos] = v;
+  }
+}
+
+// Run-time-k candidate lists over an LDS column, as a max-heap (top = the
+// k-th smallest key).
+//  SORTED = false: HeapCandidateList / FlexHeapCandidateList (knn.h:240-
+//    287), written out in heap order as the reference leaves it.
+//  SORTED = true: FixedCandidateList with a run-time k (knn.h:214-232).  The
+//    fixed list keeps the k smallest keys offered so far, ascending, and
+//    culls with the largest of them; a max-heap given the same candidates
+//    holds the same k keys with the same largest one, so every cull and
+//    every accepted candidate is the same, and one heapsort at the end
+//    gives the fixed list's ascending order.  (A sorted insert moves up to
+//    k - 1 words per accepted candidate; the heap moves log2(k).)
+template <bool SORTED> struct LdsList {
+  static constexpr bool kSorted = SORTED;
+  LdsCol e;
   int k;
-  __device__ inline float maxRadius2() const { return __uint_as_float((uint32_t)(entry[k - 1] >> 32)); }
+  uint64_t top;  // e[0]
+  __device__ inline void init_top(uint64_t v) { top = v; }
+  __device__ inline float maxRadius2() const { return __uint_as_float((uint32_t)(top >> 32)); }
   __device__ inline float initialCullDist2() const { return maxRadius2(); }
   __device__ inline float processCandidate(int id, float d2)
   {
-    uint64_t v = ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)id;
-    for (int i = 0; i < k; i++) {
-      const uint64_t lo = entry[i] < v ? entry[i] : v;
-      const uint64_t hi = entry[i] < v ? v : entry[i];
-      entry[i] = lo;
-      v = hi;
-    }
+    lds_heap_push(e, k, list_key(d2, (uint32_t)id), top);
     return maxRadius2();
   }
 };
+using DynFixedList = LdsList<true>;
+using LdsHeapList = LdsList<false>;
 
 // persistent per-query candidate lists in HBM (multi-round kNN)
 struct ListArgs {
@@ -86,17 +169,18 @@ struct ListArgs {
 
 // The reference's global-memory FlexHeapCandidateList of a multi-round
 // query (samples/mpiHugeQuery.cu:75-88, knn.h:360-421): the k entries are
-// loaded from HBM into per-lane storage, updated with the same heap push,
+// loaded from HBM into the LDS column, updated with the same heap push,
 // and stored back, so they end as the reference's in-place list would.
 struct AccumList {
-  uint64_t mem[64];
+  LdsCol e;
   int k;
   int id_offset;
-  __device__ inline float maxRadius2() const { return __uint_as_float((uint32_t)(mem[0] >> 32)); }
+  uint64_t top;
+  __device__ inline float maxRadius2() const { return __uint_as_float((uint32_t)(top >> 32)); }
   __device__ inline float initialCullDist2() const { return maxRadius2(); }
   __device__ inline float processCandidate(int id, float d2)
   {
-    cukd::detail::heap_push(mem, k, ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)(id + id_offset));
+    lds_heap_push(e, k, list_key(d2, (uint32_t)(id + id_offset)), top);
     return maxRadius2();
   }
 };
@@ -104,30 +188,38 @@ struct AccumList {
 // ---------------------------------------------------------------------------
 // per-result-type set-up and output
 // ---------------------------------------------------------------------------
+// kLds: the type keeps k 64-bit words per query in the kernel's dynamic
+// LDS (init gets the lane's column; the others ignore it)
 template <class R> struct ResultOps;
 
 template <> struct ResultOps<AccumList> {
+  static constexpr bool kLds = true;
   struct State { AccumList r; };
-  __device__ static inline AccumList &init(State &s, float cutoff, int k, int q, const ListArgs &la)
+  __device__ static inline AccumList &init(State &s, float cutoff, int k, int q, const ListArgs &la,
+                                           LdsCol col)
   {
+    s.r.e = col;
     s.r.k = k;
     s.r.id_offset = la.id_offset;
     const uint64_t *src = la.lists + (size_t)q * k;
-    const uint64_t v = ((uint64_t)__float_as_uint(cutoff * cutoff) << 32) | 0xffffffffu;
-    for (int i = 0; i < k; i++) s.r.mem[i] = la.init ? v : src[i];
+    const uint64_t v = list_key(cutoff * cutoff, 0xffffffffu);
+    for (int i = 0; i < k; i++) col[i] = la.init ? v : src[i];
+    s.r.top = col[0];
     return s.r;
   }
   __device__ static inline void write(const State &s, int q, int k, int32_t *, float *, float *,
                                       const ListArgs &la)
   {
     uint64_t *dst = la.lists + (size_t)q * k;
-    for (int i = 0; i < k; i++) dst[i] = s.r.mem[i];
+    for (int i = 0; i < k; i++) dst[i] = s.r.e[i];
   }
 };
 
 template <> struct ResultOps<cukd::FCPResult> {
+  static constexpr bool kLds = false;
   struct State { cukd::FCPResult r; };
-  __device__ static inline cukd::FCPResult &init(State &s, float cutoff, int, int, const ListArgs &)
+  __device__ static inline cukd::FCPResult &init(State &s, float cutoff, int, int, const ListArgs &,
+                                                 LdsCol)
   {
     s.r.clear(cutoff * cutoff);
     return s.r;
@@ -141,9 +233,10 @@ template <> struct ResultOps<cukd::FCPResult> {
 };
 
 template <int K> struct ResultOps<cukd::FixedCandidateList<K>> {
+  static constexpr bool kLds = false;
   struct State { cukd::FixedCandidateList<K> r; __device__ State() : r(0.f) {} };
   __device__ static inline cukd::FixedCandidateList<K> &init(State &s, float cutoff, int, int,
-                                                             const ListArgs &)
+                                                             const ListArgs &, LdsCol)
   {
     s.r = cukd::FixedCandidateList<K>(cutoff);
     return s.r;
@@ -160,50 +253,33 @@ template <int K> struct ResultOps<cukd::FixedCandidateList<K>> {
   }
 };
 
-template <> struct ResultOps<DynFixedList> {
-  struct State { DynFixedList r; };
-  __device__ static inline DynFixedList &init(State &s, float cutoff, int k, int, const ListArgs &)
+// the run-time-k fixed and heap lists: (cutoff^2, -1) in every entry, and
+// the entries written out in storage order
+template <class L> struct LdsListOps {
+  static constexpr bool kLds = true;
+  struct State { L r; };
+  __device__ static inline L &init(State &s, float cutoff, int k, int, const ListArgs &, LdsCol col)
   {
+    s.r.e = col;
     s.r.k = k;
-    const uint64_t v = ((uint64_t)__float_as_uint(cutoff * cutoff) << 32) | 0xffffffffu;
-    for (int i = 0; i < k; i++) s.r.entry[i] = v;
+    const uint64_t v = list_key(cutoff * cutoff, 0xffffffffu);
+    for (int i = 0; i < k; i++) col[i] = v;
+    s.r.init_top(v);
     return s.r;
   }
   __device__ static inline void write(const State &s, int q, int k, int32_t *ids, float *dist2, float *ret,
                                       const ListArgs &)
   {
+    if (L::kSorted) lds_heap_sort(s.r.e, k);
     for (int i = 0; i < k; i++) {
-      ids[(size_t)q * k + i] = (int32_t)(uint32_t)s.r.entry[i];
-      dist2[(size_t)q * k + i] = __uint_as_float((uint32_t)(s.r.entry[i] >> 32));
+      const uint64_t x = s.r.e[i];
+      ids[(size_t)q * k + i] = (int32_t)(uint32_t)x;
+      dist2[(size_t)q * k + i] = __uint_as_float((uint32_t)(x >> 32));
     }
     if (ret) ret[q] = s.r.maxRadius2();
   }
 };
-
-// HeapCandidateList with a run-time k: the reference's flexible heap list
-// over per-lane storage
-template <> struct ResultOps<cukd::FlexHeapCandidateList> {
-  struct State {
-    uint64_t mem[64];
-    cukd::FlexHeapCandidateList r;
-    __device__ State() : r(mem, 1, -1.f) {}
-  };
-  __device__ static inline cukd::FlexHeapCandidateList &init(State &s, float cutoff, int k, int,
-                                                             const ListArgs &)
-  {
-    new (&s.r) cukd::FlexHeapCandidateList(s.mem, k, cutoff);
-    return s.r;
-  }
-  __device__ static inline void write(const State &s, int q, int k, int32_t *ids, float *dist2, float *ret,
-                                      const ListArgs &)
-  {
-    for (int i = 0; i < k; i++) {
-      ids[(size_t)q * k + i] = (int32_t)(uint32_t)s.mem[i];
-      dist2[(size_t)q * k + i] = __uint_as_float((uint32_t)(s.mem[i] >> 32));
-    }
-    if (ret) ret[q] = s.r.maxRadius2();
-  }
-};
+template <bool SORTED> struct ResultOps<LdsList<SORTED>> : LdsListOps<LdsList<SORTED>> {};
 
 // Queries over a forest (many trees back to back in one array): query qi
 // belongs to the segment s with seg_off[s] <= qi < seg_off[s+1] and runs
@@ -257,8 +333,9 @@ __global__ __launch_bounds__(256) void k_query(TreeView t, const float *queries,
   float q[D];
 #pragma unroll
   for (int d = 0; d < D; d++) q[d] = queries[(size_t)qi * D + d];
+  HIP_DYNAMIC_SHARED(uint64_t, s_lists);
   typename ResultOps<R>::State st;
-  R &res = ResultOps<R>::init(st, cutoff, k, qi, la);
+  R &res = ResultOps<R>::init(st, cutoff, k, qi, la, LdsCol{s_lists + threadIdx.x, (int)blockDim.x});
   unsigned visits = 0;
   TreeViewD<D> tv;
   static_cast<TreeView &>(tv) = t;
@@ -420,8 +497,8 @@ __global__ __launch_bounds__(256) void k_query2(TreeView t, const float *queries
     pqb[d] = queries[(size_t)qb * D + d];
   }
   typename ResultOps<R>::State sta, stb;
-  R &ra = ResultOps<R>::init(sta, cutoff, k, qa, la);
-  R &rb = ResultOps<R>::init(stb, cutoff, k, qb, la);
+  R &ra = ResultOps<R>::init(sta, cutoff, k, qa, la, LdsCol{});
+  R &rb = ResultOps<R>::init(stb, cutoff, k, qb, la, LdsCol{});
   SbSlot sa{n_a > 0 ? SB_DESCEND : SB_DONE, 0, 0, 0, 0u, 0u, ra.initialCullDist2()};
   SbSlot sb{(has_b && n_b > 0) ? SB_DESCEND : SB_DONE, 0, 0, 0, 0u, 0u, rb.initialCullDist2()};
   unsigned visits = 0;
@@ -568,8 +645,8 @@ __global__ __launch_bounds__(256) void k_query2s(TreeView t, const float *querie
     pqb[d] = queries[(size_t)qb * D + d];
   }
   typename ResultOps<R>::State sta, stb;
-  R &ra = ResultOps<R>::init(sta, cutoff, k, qa, la);
-  R &rb = ResultOps<R>::init(stb, cutoff, k, qb, la);
+  R &ra = ResultOps<R>::init(sta, cutoff, k, qa, la, LdsCol{});
+  R &rb = ResultOps<R>::init(stb, cutoff, k, qb, la, LdsCol{});
   SdSlot sa{n_a > 0 ? SB_DESCEND : SB_DONE, 0, 0, 0u, 0u, 0.f, ra.initialCullDist2()};
   SdSlot sb{(has_b && n_b > 0) ? SB_DESCEND : SB_DONE, 0, 0, 0u, 0u, 0.f, rb.initialCullDist2()};
   unsigned visits = 0;
@@ -682,7 +759,11 @@ inline hipError_t launch_q(const QueryArgs &a, hipStream_t s)
     }
   }
   const int blocks = (a.nq + nt - 1) / nt;
-  hipLaunchKernelGGL((k_query<D, METHOD, R>), dim3(blocks), dim3(nt), 0, s, a.t, a.queries,
+  // run-time-k lists: k words per lane of dynamic LDS (k <= 64 and nt <=
+  // 256 keep it within one CU's 160 KiB)
+  const size_t shm = ResultOps<R>::kLds ? (size_t)a.k * sizeof(uint64_t) * nt : 0;
+  if (shm > LDS_PER_CU) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_query<D, METHOD, R>), dim3(blocks), dim3(nt), shm, s, a.t, a.queries,
                      a.nq, a.world_bounds, a.cutoff, a.eps, a.k, a.order, a.ids, a.dist2,
                      a.ret, a.la, a.fq, a.stats);
   return hipGetLastError();
@@ -692,7 +773,7 @@ template <int D, int METHOD>
 inline hipError_t launch_method(const QueryArgs &a, hipStream_t s)
 {
   if (a.list_kind < 0) return launch_q<D, METHOD, cukd::FCPResult>(a, s);
-  if (a.list_kind == CUKD_LIST_HEAP) return launch_q<D, METHOD, cukd::FlexHeapCandidateList>(a, s);
+  if (a.list_kind == CUKD_LIST_HEAP) return launch_q<D, METHOD, LdsHeapList>(a, s);
   if (a.list_kind == LIST_ACCUM) return launch_q<D, METHOD, AccumList>(a, s);
   switch (a.k) {
     case 1: return launch_q<D, METHOD, cukd::FixedCandidateList<1>>(a, s);

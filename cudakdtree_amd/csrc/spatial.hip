@@ -550,9 +550,11 @@ __global__ __launch_bounds__(128) void k_spatial_query(const cukd_spatial_node *
   float q[D];
 #pragma unroll
   for (int d = 0; d < D; d++) q[d] = queries[(size_t)i * D + d];
+  HIP_DYNAMIC_SHARED(uint64_t, s_lists);
   typename ResultOps<R>::State st;
   const ListArgs no_lists{nullptr, 0, 0};
-  R &res = ResultOps<R>::init(st, cutoff, k, i, no_lists);
+  R &res = ResultOps<R>::init(st, cutoff, k, i, no_lists,
+                              LdsCol{s_lists + threadIdx.x, (int)blockDim.x});
   float lo[D], hi[D];
 #pragma unroll
   for (int d = 0; d < D; d++) { lo[d] = bounds.v[d]; hi[d] = bounds.v[D + d]; }
@@ -573,12 +575,14 @@ template <int D, bool CCT>
 static hipError_t launch_spatial_m(const SpatialArgs &a, hipStream_t s)
 {
   const int nt = 128, blocks = (a.nq + nt - 1) / nt;
+  // run-time-k lists keep k words per lane in dynamic LDS (64 KiB at k = 64)
 #define CUKD_SPQ(R)                                                                          \
-  hipLaunchKernelGGL((k_spatial_query<D, CCT, R>), dim3(blocks), dim3(nt), 0, s, a.nodes,   \
+  hipLaunchKernelGGL((k_spatial_query<D, CCT, R>), dim3(blocks), dim3(nt),                  \
+                     ResultOps<R>::kLds ? (size_t)a.k * sizeof(uint64_t) * nt : 0, s, a.nodes, \
                      a.prim_ids, a.data, a.bounds, a.queries, a.nq, a.cutoff, a.k, a.ids,   \
                      a.dist2, a.ret, a.stats)
   if (a.list_kind < 0) CUKD_SPQ(cukd::FCPResult);
-  else if (a.list_kind == CUKD_LIST_HEAP) CUKD_SPQ(cukd::FlexHeapCandidateList);
+  else if (a.list_kind == CUKD_LIST_HEAP) CUKD_SPQ(LdsHeapList);
   else if (a.k == 4) CUKD_SPQ(cukd::FixedCandidateList<4>);
   else if (a.k == 8) CUKD_SPQ(cukd::FixedCandidateList<8>);
   else if (a.k == 16) CUKD_SPQ(cukd::FixedCandidateList<16>);

@@ -108,7 +108,8 @@ bool build_case(const char *dist, int n, const cukd_layout &L, int builder, uint
   return ok;
 }
 
-bool query_case(int n, int nq, int D, int method, int k, int flags, uint64_t seed)
+bool query_case(int n, int nq, int D, int method, int k, int flags, uint64_t seed,
+                int list = CUKD_LIST_FIXED)
 {
   cukd_layout L{D, CUKD_SCALAR_F32, 4 * D, 0, -1, 4};
   std::vector<uint32_t> e = points(n, L, seed, "coarse");
@@ -136,12 +137,12 @@ bool query_case(int n, int nq, int D, int method, int k, int flags, uint64_t see
     rc = cukd_fcp(method, dt, n, &L, db, (const float *)dq, nq, INFINITY, 0.f, (int32_t *)di,
                   (float *)dd, flags, nullptr, nullptr);
   } else {
-    oc_knn(method, OC_LIST_FIXED, k, e.data(), n, &O, box.data(), q.data(), nq, INFINITY,
+    oc_knn(method, list, k, e.data(), n, &O, box.data(), q.data(), nq, INFINITY,
            rid.data(), rd2.data(), rr.data(), nullptr, 1);
-    rc = (flags & 64) ? cukd_knn_noalloc(method, CUKD_LIST_FIXED, k, dt, n, &L, db,
+    rc = (flags & 64) ? cukd_knn_noalloc(method, list, k, dt, n, &L, db,
                                           (const float *)dq, nq, INFINITY, (int32_t *)di,
                                           (float *)dd, (float *)dr, flags & ~64, nullptr)
-                      : cukd_knn(method, CUKD_LIST_FIXED, k, dt, n, &L, db, (const float *)dq,
+                      : cukd_knn(method, list, k, dt, n, &L, db, (const float *)dq,
                                  nq, INFINITY, (int32_t *)di, (float *)dd, (float *)dr, flags,
                                  nullptr, nullptr);
     memcpy(gr.data(), dr, rr.size() * 4);
@@ -207,7 +208,59 @@ bool forest_case(const std::vector<int64_t> &off, uint64_t seed, int per = 40)
   return ok;
 }
 
-bool spatial_case(int n, int D, int scalar, uint64_t seed)
+// two shards into one set of persistent heap lists, against the oracle's
+// lists word for word, then the extraction
+bool accumulate_case(int n, int nq, int k, int method, uint64_t seed)
+{
+  cukd_layout L{3, CUKD_SCALAR_F32, 12, 0, -1, 4};
+  const oc_layout O = ocl(L);
+  std::vector<uint32_t> s0 = points(n, L, seed, "coarse"), s1 = points(n / 2 + 1, L, seed + 7, "coarse");
+  std::vector<float> b0(6), b1(6);
+  oc_build_host(s0.data(), n, &O, b0.data());
+  oc_build_host(s1.data(), n / 2 + 1, &O, b1.data());
+  std::vector<float> q = queries(nq, 3, seed + 1);
+  std::vector<uint64_t> rl((size_t)nq * k), gl((size_t)nq * k);
+  oc_knn_accumulate(method, k, s0.data(), n, &O, b0.data(), q.data(), nq, rl.data(), 1, 0.3f, 0,
+                    nullptr, 1);
+  oc_knn_accumulate(method, k, s1.data(), n / 2 + 1, &O, b1.data(), q.data(), nq, rl.data(), 0,
+                    0.3f, n, nullptr, 1);
+  void *d0, *d1, *db0, *db1, *dq, *dl, *di, *dd, *dr;
+  hipMalloc(&d0, s0.size() * 4);
+  hipMalloc(&d1, s1.size() * 4);
+  hipMalloc(&db0, 24);
+  hipMalloc(&db1, 24);
+  hipMalloc(&dq, q.size() * 4);
+  hipMalloc(&dl, rl.size() * 8);
+  hipMalloc(&di, (size_t)nq * k * 4);
+  hipMalloc(&dd, (size_t)nq * k * 4);
+  hipMalloc(&dr, (size_t)nq * 4);
+  memcpy(d0, s0.data(), s0.size() * 4);
+  memcpy(d1, s1.data(), s1.size() * 4);
+  memcpy(db0, b0.data(), 24);
+  memcpy(db1, b1.data(), 24);
+  memcpy(dq, q.data(), q.size() * 4);
+  int rc = cukd_knn_accumulate(method, k, d0, n, &L, db0, (const float *)dq, nq, (uint64_t *)dl, 1,
+                               0.3f, 0, 0, nullptr, nullptr);
+  if (rc == CUKD_OK)
+    rc = cukd_knn_accumulate(method, k, d1, n / 2 + 1, &L, db1, (const float *)dq, nq,
+                             (uint64_t *)dl, 0, 0.3f, n, 0, nullptr, nullptr);
+  memcpy(gl.data(), dl, gl.size() * 8);
+  bool ok = rc == CUKD_OK && gl == rl;
+  if (ok) {
+    std::vector<int32_t> rid((size_t)nq * k);
+    std::vector<float> rd2((size_t)nq * k), rr(nq);
+    oc_knn_extract(rl.data(), nq, k, rid.data(), rd2.data(), rr.data());
+    rc = cukd_knn_extract((const uint64_t *)dl, nq, k, (int32_t *)di, (float *)dd, (float *)dr,
+                          nullptr);
+    ok = rc == CUKD_OK && !memcmp(di, rid.data(), rid.size() * 4) &&
+         !memcmp(dd, rd2.data(), rd2.size() * 4) && !memcmp(dr, rr.data(), rr.size() * 4);
+  }
+  if (rc != CUKD_OK) printf("  accumulate failed: %s\n", cukd_last_error());
+  for (void *p : {d0, d1, db0, db1, dq, dl, di, dd, dr}) hipFree(p);
+  return ok;
+}
+
+bool spatial_case(int n, int D, int scalar, uint64_t seed, int k = 8, int list = CUKD_LIST_FIXED)
 {
   cukd_layout L{D, scalar, 4 * D, 0, -1, 4};
   std::vector<uint32_t> e = points(n, L, seed, "coarse");
@@ -226,18 +279,18 @@ bool spatial_case(int n, int D, int scalar, uint64_t seed)
             !memcmp(t.prim_ids, rp.data(), (size_t)n * 4) &&
             !memcmp(t.ibounds, rb.data(), rb.size() * 4);
   if (ok && scalar == CUKD_SCALAR_F32) {
-    const int nq = 500, k = 8;
+    const int nq = 500;
     std::vector<float> q = queries(nq, D, seed + 3);
     std::vector<int32_t> rid(nq * k), gid(nq * k);
     std::vector<float> rd2(nq * k), gd2(nq * k);
-    oc_spatial_knn(OC_STACK_BASED, OC_LIST_FIXED, k, rn.data(), rp.data(), (float *)rb.data(),
+    oc_spatial_knn(OC_STACK_BASED, list, k, rn.data(), rp.data(), (float *)rb.data(),
                    e.data(), &O, q.data(), nq, 0.2f, rid.data(), rd2.data(), nullptr, 1);
     void *dq, *di, *dd;
     hipMalloc(&dq, q.size() * 4);
     hipMalloc(&di, gid.size() * 4);
     hipMalloc(&dd, gd2.size() * 4);
     memcpy(dq, q.data(), q.size() * 4);
-    rc = cukd_spatial_knn(CUKD_STACK_BASED, CUKD_LIST_FIXED, k, &t, &L, (const float *)dq, nq,
+    rc = cukd_spatial_knn(CUKD_STACK_BASED, list, k, &t, &L, (const float *)dq, nq,
                           0.2f, (int32_t *)di, (float *)dd, nullptr, nullptr);
     ok = rc == CUKD_OK && !memcmp(di, rid.data(), gid.size() * 4) &&
          !memcmp(dd, rd2.data(), gd2.size() * 4);
@@ -320,6 +373,18 @@ int main(int argc, char **argv)
     for (int k : {0, 1, 8, 20})
       cases.push_back({"query method=" + std::to_string(m) + " k=" + std::to_string(k),
                        [=] { return query_case(6000, 3000, 3, m, k, 0, 41 + m * 10 + k); }});
+  // run-time-k lists (entries in LDS columns): fixed k = 20 / 64, heap k = 5 / 64
+  for (int m : {CUKD_STACK_BASED, CUKD_STACK_FREE, CUKD_CCT}) {
+    cases.push_back({"query method=" + std::to_string(m) + " fixed k=64", [=] {
+                       return query_case(3000, 700, 3, m, 64, 0, 111 + m);
+                     }});
+    for (int k : {5, 64})
+      cases.push_back({"query method=" + std::to_string(m) + " heap k=" + std::to_string(k), [=] {
+                         return query_case(3000, 700, 3, m, k, 0, 121 + m * 3 + k, CUKD_LIST_HEAP);
+                       }});
+    cases.push_back({"accumulate method=" + std::to_string(m) + " k=20",
+                     [=] { return accumulate_case(2500, 600, 20, m, 131 + m); }});
+  }
   cases.push_back({"query kNN-8 Morton-sorted", [=] {
                      return query_case(6000, 3000, 3, CUKD_STACK_BASED, 8, CUKD_QUERY_SORT, 47);
                    }});
@@ -362,6 +427,11 @@ int main(int argc, char **argv)
                    }});
   cases.push_back({"spatial float3 build + kNN-8", [=] { return spatial_case(6000, 3, CUKD_SCALAR_F32, 67); }});
   cases.push_back({"spatial int4 build", [=] { return spatial_case(6000, 4, CUKD_SCALAR_I32, 71); }});
+  cases.push_back({"spatial float3 kNN-20",
+                   [=] { return spatial_case(4000, 3, CUKD_SCALAR_F32, 141, 20); }});
+  cases.push_back({"spatial float3 heap kNN-33", [=] {
+                     return spatial_case(4000, 3, CUKD_SCALAR_F32, 143, 33, CUKD_LIST_HEAP);
+                   }});
   for (auto &c : cases)
     if (only.empty() || c.name.find(only) != std::string::npos) report(c.name, c.run());
   printf(g_fail ? "%d FAILED\n" : "ALL OK\n", g_fail);

@@ -72,6 +72,18 @@ def test_knn_k_values(k):
     check_knn(*s, q, k, float("inf"))
 
 
+@pytest.mark.parametrize("k", [20, 64])
+def test_knn_run_time_k_lists_sorted_and_autotuned(k):
+    """The run-time-k lists keep their entries in LDS columns, one column
+    per lane, the workgroup size apart: Morton-ordered launches and the
+    autotuner's 64/128/256-lane workgroups give the oracle's lists too."""
+    coords = distribution("gmm", 40000, 3, seed=k + 5)
+    s = _setup(coords)
+    q = datagen.uniform(5000, 3, k + 6)
+    check_knn(*s, q, k, float("inf"), flags=ck.QUERY_SORT)
+    check_knn(*s, q, k, 0.1, flags=ck.QUERY_AUTOTUNE)
+
+
 @pytest.mark.parametrize("D", [2, 3, 4, 8])
 @pytest.mark.parametrize("dist", ["ties", "gmm", "dups"])
 def test_knn_ties_and_cutoff(D, dist):
