@@ -560,6 +560,28 @@ def spatial_bench(args, ck, datagen, torch, np):
         torch.cuda.synchronize()
         if i >= 1:
             qms.append(e0.elapsed_time(e1))
+    # the reference's spatial sample keeps a HeapCandidateList<16>
+    # (samples/knn-float3-spatialkdtree.cu:26): the heap list (LDS columns),
+    # whose sorted keys must equal the fixed list's
+    hids = torch.empty_like(ids)
+    hd2 = torch.empty_like(d2)
+    hms = []
+    for i in range(6):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        rc = lib.cukd_spatial_knn(ck.STACK_BASED, ck.LIST_HEAP, k, ctypes.byref(tree.raw),
+                                  ctypes.byref(lay), ctypes.c_void_p(q.data_ptr()), nq, r,
+                                  ctypes.c_void_p(hids.data_ptr()), ctypes.c_void_p(hd2.data_ptr()),
+                                  ctypes.c_void_p(ret.data_ptr()), ctypes.c_void_p(s.cuda_stream))
+        assert rc == 0, lib.cukd_last_error()
+        e1.record(s)
+        torch.cuda.synchronize()
+        if i >= 1:
+            hms.append(e0.elapsed_time(e1))
+    key = lambda a, b: (b.view(torch.int32).to(torch.int64) << 32) | (a.to(torch.int64) & 0xFFFFFFFF)
+    heap_ok = torch.equal(torch.sort(key(hids, hd2), dim=1).values, key(ids, d2))
+    del hids, hd2
     # int3 set, build only
     ni = 2_000_000
     ci = (datagen.splitmix64(17, ni * 3) >> np.uint64(44)).astype(np.int32).reshape(ni, 3)
@@ -577,10 +599,12 @@ def spatial_bench(args, ck, datagen, torch, np):
         itree = ck.buildSpatialTree(dci, ilay)
         ims.append((time.perf_counter() - t0) * 1e3)
     out = {"workload": "spatial k-d tree: 4M float4 U[-1,1)^4 (1% duplicates) build + kNN k=16 "
-                       "r=0.05 (stack-based, fixed list) of 2M queries; 2M int3 build",
+                       "r=0.05 (stack-based, fixed and heap lists) of 2M queries; 2M int3 build",
            "gpu_build_ms_4M_float4": round(min(bms[1:]), 3),
            "gpu_knn16_ms_2M": round(_pct(qms, 50), 3),
            "gpu_knn16_queries_per_s": round(nq / (_pct(qms, 50) / 1e3), 1),
+           "gpu_knn16_heap_ms_2M": round(_pct(hms, 50), 3),
+           "knn16_heap_sorted_equals_fixed": heap_ok,
            "gpu_build_ms_2M_int3": round(min(ims[1:]), 3), "nodes_4M": tree.num_nodes}
     if not args.no_cpu:
         import oracle as oc
