@@ -537,6 +537,24 @@ int spatial_build_impl(cukd_spatial_tree *tree, const void *d_points, int n, con
 // spatial queries
 // ---------------------------------------------------------------------------
 
+// The stack-based query with a register-resident result keeps the first
+// SP_STACK_COLS entries of its traversal stack in LDS columns: the stack
+// is indexed at run time, so a per-lane array lives in scratch.  The
+// 0.05-radius queries of the spatial config rarely go deeper.  (kNN-16 of
+// 2M queries over 4M float4 points: 15.9 -> 12.3 ms.)  A list already in
+// LDS keeps the whole stack in scratch: with both, the 32 KiB per
+// workgroup cost more occupancy than the stack gained (heap kNN-16
+// 14.5 -> 16.3 ms).
+template <bool CCT, class R> struct SpStackCols {
+  enum { value = (CCT || ResultOps<R>::kLds) ? 0 : 16 };
+};
+
+template <bool CCT, class R>
+__host__ __device__ constexpr size_t sp_query_lds_words(int k, int nt)
+{
+  return (ResultOps<R>::kLds ? (size_t)k * 2 * nt : 0) + (size_t)SpStackCols<CCT, R>::value * 2 * nt;
+}
+
 template <int D, bool CCT, class R>
 __global__ __launch_bounds__(128) void k_spatial_query(const cukd_spatial_node *nodes,
                                                        const uint32_t *prim_ids, TreeView data,
@@ -559,8 +577,14 @@ __global__ __launch_bounds__(128) void k_spatial_query(const cukd_spatial_node *
 #pragma unroll
   for (int d = 0; d < D; d++) { lo[d] = bounds.v[d]; hi[d] = bounds.v[D + d]; }
   unsigned visits = 0;
-  const bool ok =
-      cukd::detail::spatial_traverse<D, CCT>(res, q, nodes, prim_ids, lo, hi, data, visits);
+  // stack columns: COLS node words, then as many dist2 words, each row
+  // blockDim.x wide
+  constexpr int COLS = SpStackCols<CCT, R>::value;
+  uint32_t *stk = (uint32_t *)s_lists + threadIdx.x;
+  const cukd::detail::SpatialStackCols cols{
+      (int *)stk, (float *)(stk + (size_t)COLS * blockDim.x), (int)blockDim.x};
+  const bool ok = cukd::detail::spatial_traverse<D, CCT, COLS>(res, q, nodes, prim_ids, lo, hi,
+                                                                data, visits, cols);
   if (stats) atomicAdd(stats, (unsigned long long)visits);
   ResultOps<R>::write(st, i, k, ids, dist2, ret, no_lists);
   // stack overflow: fcp reports no result, knn returns -1 (fcp.h:317-320,
@@ -575,10 +599,11 @@ template <int D, bool CCT>
 static hipError_t launch_spatial_m(const SpatialArgs &a, hipStream_t s)
 {
   const int nt = 128, blocks = (a.nq + nt - 1) / nt;
-  // run-time-k lists keep k words per lane in dynamic LDS (64 KiB at k = 64)
+  // dynamic LDS: k words per lane for run-time-k lists (64 KiB at k = 64),
+  // or the stack-based traversal's first 16 stack entries (16 KiB)
 #define CUKD_SPQ(R)                                                                          \
   hipLaunchKernelGGL((k_spatial_query<D, CCT, R>), dim3(blocks), dim3(nt),                  \
-                     ResultOps<R>::kLds ? (size_t)a.k * sizeof(uint64_t) * nt : 0, s, a.nodes, \
+                     (sp_query_lds_words<CCT, R>(a.k, nt) * sizeof(uint32_t)), s, a.nodes,  \
                      a.prim_ids, a.data, a.bounds, a.queries, a.nq, a.cutoff, a.k, a.ids,   \
                      a.dist2, a.ret, a.stats)
   if (a.list_kind < 0) CUKD_SPQ(cukd::FCPResult);

@@ -45,19 +45,32 @@ struct SpatialKDTree {
 
 namespace detail {
 
+// Caller-provided storage for the first entries of the stack-based
+// traversal's stack: entry i at node[i * stride] / d2[i * stride] (for
+// example one column per lane in shared memory, so the entries a kernel
+// touches most are not in per-lane scratch).
+struct SpatialStackCols {
+  int *node;
+  float *d2;
+  int stride;
+};
+
 // Stack-based (CCT = false) or closest-corner (CCT = true) traversal of a
 // spatial tree with the reference's 50-entry stack; returns false when
 // that stack overflows (the reference prints "STACK OVERFLOW" and returns
-// -1, fcp.h:317-320, knn.h:620-623).
-template <int D, bool CCT, class R, class NodeT, class Acc>
+// -1, fcp.h:317-320, knn.h:620-623).  With COLS > 0 (stack-based only),
+// entries 0 .. COLS-1 live in 'cols' and the rest in the local array.
+template <int D, bool CCT, int COLS = 0, class R, class NodeT, class Acc>
 inline __both__ bool spatial_traverse(R &result, const float (&q)[D], const NodeT *nodes,
                                       const uint32_t *prim_ids, const float (&lo)[D],
-                                      const float (&hi)[D], const Acc &data, unsigned &visits)
+                                      const float (&hi)[D], const Acc &data, unsigned &visits,
+                                      SpatialStackCols cols = SpatialStackCols{nullptr, nullptr, 0})
 {
-  enum { DEPTH = 50 };
+  enum { DEPTH = 50, LOCAL = CCT ? DEPTH : DEPTH - COLS };
+  static_assert(COLS >= 0 && COLS < DEPTH && (COLS == 0 || !CCT), "stack columns: stack-based only");
   float cull = result.initialCullDist2();
-  int st_node[DEPTH];
-  float st_d2[CCT ? 1 : DEPTH];
+  int st_node[LOCAL];
+  float st_d2[CCT ? 1 : LOCAL];
   float st_corner[CCT ? DEPTH : 1][D];
   int sp = 0;
   int node_id = 0;
@@ -99,8 +112,13 @@ inline __both__ bool spatial_traverse(R &result, const float (&q)[D], const Node
         const float diff = qc - pos;
         const float plane2 = diff * diff;
         if (plane2 < cull) {
-          st_node[sp] = far_c;
-          st_d2[CCT ? 0 : sp] = plane2;
+          if (COLS > 0 && sp < COLS) {
+            cols.node[sp * cols.stride] = far_c;
+            cols.d2[sp * cols.stride] = plane2;
+          } else {
+            st_node[sp - COLS] = far_c;
+            st_d2[CCT ? 0 : sp - COLS] = plane2;
+          }
           if (++sp >= DEPTH) return false;
         }
       }
@@ -120,10 +138,12 @@ inline __both__ bool spatial_traverse(R &result, const float (&q)[D], const Node
 #pragma unroll
         for (int d = 0; d < D; d++) closest[d] = st_corner[CCT ? sp : 0][d];
         if (sqr_dist<D>(closest, q) >= cull) continue;
-      } else if (st_d2[CCT ? 0 : sp] >= cull) {
-        continue;
+        node_id = st_node[sp];
+        break;
       }
-      node_id = st_node[sp];
+      const bool in_cols = COLS > 0 && sp < COLS;
+      if ((in_cols ? cols.d2[sp * cols.stride] : st_d2[CCT ? 0 : sp - COLS]) >= cull) continue;
+      node_id = in_cols ? cols.node[sp * cols.stride] : st_node[sp - COLS];
       break;
     }
   }

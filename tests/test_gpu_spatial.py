@@ -69,6 +69,33 @@ def test_spatial_queries(method, dist):
     ck.free(t)
 
 
+def test_spatial_deep_stack_beyond_lds_entries():
+    """The batched stack-based query keeps its first 16 stack entries in
+    LDS and the rest in a per-lane array.  Unbounded-radius queries push a
+    far child at every level of their first descent, and the leaves of this
+    1M-point tree lie at depth 21 (median; 6 to 26), so both parts are
+    used."""
+    D = 3
+    coords = distribution("gmm", 1_000_000, D, seed=21)
+    lay, (rn, rp, rb), t = build_both(coords)
+    q = datagen.uniform(1500, D, 22)
+    dq = torch.from_numpy(q).cuda()
+    cutoff = float("inf")
+    gid, gd2 = ck.spatial_fcp(t, dq, ck.STACK_BASED, cutoff)
+    rid, rd2 = oc.spatial_fcp(oc.STACK_BASED, rn, rp, rb, coords, lay, q, cutoff)
+    torch.cuda.synchronize()
+    assert np.array_equal(gid.cpu().numpy(), rid)
+    assert np.array_equal(gd2.cpu().numpy().view(np.uint32), rd2.view(np.uint32))
+    for lk, k in ((ck.LIST_FIXED, 4), (ck.LIST_FIXED, 16), (ck.LIST_HEAP, 16), (ck.LIST_FIXED, 33)):
+        gid, gd2, gr = ck.spatial_knn(t, dq, k, ck.STACK_BASED, lk, cutoff)
+        rid, rd2, rr = oc.spatial_knn(oc.STACK_BASED, lk, k, rn, rp, rb, coords, lay, q, cutoff)
+        torch.cuda.synchronize()
+        assert np.array_equal(gid.cpu().numpy(), rid), (lk, k)
+        assert np.array_equal(gd2.cpu().numpy().view(np.uint32), rd2.view(np.uint32)), (lk, k)
+        assert np.array_equal(gr.cpu().numpy().view(np.uint32), rr.view(np.uint32)), (lk, k)
+    ck.free(t)
+
+
 def test_spatial_knn16_cutoff_config_shape():
     """4M float4 in [-1,1)^4 with 1% duplicates; kNN k=16, r=0.05 on queries
     half from the data (+N(0,0.001) jitter) and half uniform -- subset of the
