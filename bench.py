@@ -453,6 +453,61 @@ def lists_bench(args, ck, torch, tree, lay, queries):
     return out
 
 
+def multiround_bench(args, ck, torch, pts, lay, queries, tree):
+    """Multi-round kNN (samples/mpiHugeQuery.cu:75-103): the 10M points
+    split into 4 contiguous shards of 2.5M, each built as its own tree (one
+    GPU's share of a 4-rank ring, every shard visiting in turn), and 4
+    rounds of stack-free kNN-8 of the 1M queries into persistent heap lists
+    in HBM (Morton-ordered launches, global ids = shard offset + index),
+    then the extraction.  Full-size checks: the extracted dist2 equal the
+    single-tree kNN-8 over all 10M points, and every id's point lies at
+    its reported dist2."""
+    dev = queries.device
+    nq, k, P = queries.shape[0], 8, 4
+    n = pts.shape[0]
+    off = [i * n // P for i in range(P + 1)]
+    shards = []
+    for r in range(P):
+        t = pts[off[r]:off[r + 1]].clone()
+        ck.buildTree(t, lay)
+        shards.append(t)
+    lists = ck.knn_lists(nq, k, device=dev)
+
+    def run():
+        for r in range(P):
+            ck.knn_accumulate(shards[r], lay, queries, lists, init=(r == 0), id_offset=off[r],
+                              flags=ck.QUERY_SORT)
+        return ck.knn_extract(lists)
+
+    run()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    reps = 3
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        ids, d2, rad = run()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    _, fd2, _ = ck.knn(tree, lay, queries, k, traversal=ck.STACK_FREE, flags=ck.QUERY_SORT)
+    allp = torch.cat(shards)
+    p = allp[ids.long()]
+    dd = p - queries[:, None, :]
+    own = (dd[..., 0] * dd[..., 0] + dd[..., 1] * dd[..., 1]) + dd[..., 2] * dd[..., 2]
+    ok = bool(torch.equal(d2, fd2) and (ids >= 0).all().item() and torch.equal(own, d2))
+    out = {"workload": "1M uniform float3 queries (seed 2), kNN k=8, over the 10M uniform points "
+                       "as 4 shards of 2.5M: 4 accumulate rounds (stack-free, Morton-ordered) "
+                       "+ extract",
+           "shards": P, "k": k, "ms_4_rounds_and_extract": round(ms, 3),
+           "queries_per_s": round(nq / (ms / 1e3), 1),
+           "equals_single_tree_knn": ok}
+    print("[bench] multiround: %s" % json.dumps(out), file=sys.stderr, flush=True)
+    del shards, lists, allp, p, dd, own
+    return out
+
+
 def c1_bench(args, ck, datagen, torch, np):
     """The plain method at its small size: a left-balanced tree over 100K
     float3 points uniform in [0,1)^3 (seed 1) built by buildTree, then FCP
@@ -669,6 +724,7 @@ def main():
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-lists", action="store_true")
+    ap.add_argument("--no-multiround", action="store_true")
     ap.add_argument("--no-spatial", action="store_true")
     ap.add_argument("--probe", action="store_true",
                     help="for hardware-counter runs: one 1 GiB device copy (the counters' "
@@ -918,6 +974,8 @@ def main():
     # ---------------- the other configurations' legs ----------------
     c2 = None if args.no_c2 else c2_bench(args, ck, datagen, torch, np, tree, lay, queries)
     lists = None if args.no_lists else lists_bench(args, ck, torch, tree, lay, queries)
+    multiround = (None if args.no_multiround else
+                  multiround_bench(args, ck, torch, pts, lay, queries, tree))
     c1 = None if args.no_c1 else c1_bench(args, ck, datagen, torch, np)
     spatial = None if args.no_spatial else spatial_bench(args, ck, datagen, torch, np)
     forest = None if args.no_forest else forest_bench(args, ck, datagen, torch, np)
@@ -1018,6 +1076,7 @@ def main():
             "c1": c1,
             "c2": c2,
             "lists": lists,
+            "multiround": multiround,
             "spatial": spatial,
             "forest": forest,
             "serve": serve,
