@@ -385,6 +385,14 @@ int main(int argc, char **argv)
     cases.push_back({"accumulate method=" + std::to_string(m) + " k=20",
                      [=] { return accumulate_case(2500, 600, 20, m, 131 + m); }});
   }
+  // run-time-k lists over Morton-ordered batches of odd sizes
+  cases.push_back({"query heap k=20 sorted n=3001", [=] {
+                     return query_case(6000, 3001, 3, CUKD_STACK_BASED, 20, CUKD_QUERY_SORT, 151,
+                                       CUKD_LIST_HEAP);
+                   }});
+  cases.push_back({"query fixed k=7 float2 sorted n=2049", [=] {
+                     return query_case(5000, 2049, 2, CUKD_STACK_BASED, 7, CUKD_QUERY_SORT, 153);
+                   }});
   cases.push_back({"query kNN-8 Morton-sorted", [=] {
                      return query_case(6000, 3000, 3, CUKD_STACK_BASED, 8, CUKD_QUERY_SORT, 47);
                    }});
